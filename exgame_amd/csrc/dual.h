@@ -1,0 +1,55 @@
+// Forward-mode dual number with ONE derivative direction: each lane of the
+// Jacobian kernel carries the value and the derivative along one of the 24 axes,
+// so a joint's full Jacobian is 24 lanes instead of one 25-wide carrier.
+//
+// The arithmetic rules are those of the reference's Tangent carrier
+// (crates/clifford/src/tangent.rs:145-310): a product's derivative is
+// b.d·a.v + a.d·b.v; a quotient is a·recip(b) with recip(b).d = b.d·(−v²); sqrt'
+// is 1/(√a+√a). The value part is therefore computed exactly as the 24-axis
+// carrier computes it, whichever axis a lane differentiates along.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace exg {
+
+template <class T>
+struct Dual {
+    T v, d;
+    __host__ __device__ Dual() : v(T(0)), d(T(0)) {}
+    __host__ __device__ explicit Dual(T value) : v(value), d(T(0)) {}
+    __host__ __device__ Dual(T value, T deriv) : v(value), d(deriv) {}
+};
+
+template <class T> __host__ __device__ inline Dual<T> operator+(const Dual<T>& a, const Dual<T>& b) { return {a.v + b.v, a.d + b.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator-(const Dual<T>& a, const Dual<T>& b) { return {a.v - b.v, a.d - b.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator-(const Dual<T>& a) { return {-a.v, -a.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator*(const Dual<T>& a, const Dual<T>& b) { return {a.v * b.v, b.d * a.v + a.d * b.v}; }
+template <class T> __host__ __device__ inline Dual<T> recip(const Dual<T>& b) {
+    T r = T(1) / b.v;
+    return {r, b.d * (-(r * r))};
+}
+template <class T> __host__ __device__ inline Dual<T> operator/(const Dual<T>& a, const Dual<T>& b) { return a * recip(b); }
+
+template <class T> __host__ __device__ inline T value_of(const Dual<T>& x) { return x.v; }
+__host__ __device__ inline double value_of(double x) { return x; }
+__host__ __device__ inline float value_of(float x) { return x; }
+
+__host__ __device__ inline double s_sqrt(double x) { return sqrt(x); }
+__host__ __device__ inline float s_sqrt(float x) { return sqrtf(x); }
+__host__ __device__ inline double s_sin(double x) { return sin(x); }
+__host__ __device__ inline float s_sin(float x) { return sinf(x); }
+__host__ __device__ inline double s_cos(double x) { return cos(x); }
+__host__ __device__ inline float s_cos(float x) { return cosf(x); }
+
+template <class T> __host__ __device__ inline Dual<T> s_sqrt(const Dual<T>& a) {
+    T v = s_sqrt(a.v);
+    return {v, a.d * (T(1) / (v + v))};
+}
+template <class T> __host__ __device__ inline Dual<T> s_sin(const Dual<T>& a) { return {s_sin(a.v), a.d * s_cos(a.v)}; }
+template <class T> __host__ __device__ inline Dual<T> s_cos(const Dual<T>& a) { return {s_cos(a.v), a.d * (-s_sin(a.v))}; }
+
+// Real type underneath a scalar (the seam selector compares the value).
+template <class S> struct RealOf { using type = S; };
+template <class T> struct RealOf<Dual<T>> { using type = T; };
+
+}  // namespace exg

@@ -1,0 +1,366 @@
+// Host side of the C ABI (include/exgame.h): device buffers of a world in HBM, the
+// per-step launch sequence, error mapping. Compiled by hipcc into libexgame_hip.so.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/exgame.h"
+#include "island.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(EXG_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+constexpr int MAX_ISLAND_BODIES = 8;
+constexpr int MAX_ISLAND_JOINTS = 16;
+
+}  // namespace
+
+struct exg_world {
+    int B = 0, I = 0, J = 0;
+    double G = 1.0;
+    bool gravity = true;
+    int integrator = EXG_NEWTON;
+    double tol2 = 1e-8;
+    int ns_warm = 2, ns_seed = 24;
+    int max_m = 0, max_j = 0;
+    hipStream_t stream = nullptr;
+    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2];
+    DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
+    DevBuf<long long> xacc_off;
+    int front = 0;
+    std::vector<hipEvent_t> ev;
+    int ev_used = 0;
+    bool timing = true;
+    double last_ms[2] = {0.0, 0.0};
+
+    void release() {
+        for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1]}) b->release();
+        for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
+        xacc_off.release();
+        for (auto e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+extern "C" {
+
+const char* exg_last_error(void) { return g_last_error.c_str(); }
+
+int exg_device_info_get(exg_device_info* out) {
+    if (!out) return fail(EXG_ERR_INVALID, "exg_device_info_get: null output");
+    hipDeviceProp_t p;
+    HIP_TRY(hipGetDeviceProperties(&p, 0));
+    out->compute_units = p.multiProcessorCount;
+    out->wave_size = p.warpSize;
+    out->lds_bytes_per_cu = (int64_t)p.maxSharedMemoryPerMultiProcessor;
+    out->hbm_bytes = (int64_t)p.totalGlobalMem;
+    std::snprintf(out->name, sizeof(out->name), "%s", p.gcnArchName);
+    return EXG_OK;
+}
+
+int exg_world_create(int n_bodies, const double* pose, const double* momentum, const double* mass,
+                     const double* inertia, const double* size, int n_islands, const int* island_start,
+                     const double* origin, int n_joints, const int* kind, const int* a, const int* b,
+                     const double* params, double G, int gravity_on, exg_world** out) {
+    if (!out) return fail(EXG_ERR_INVALID, "exg_world_create: null output");
+    *out = nullptr;
+    if (n_bodies <= 0 || n_islands <= 0 || n_joints < 0)
+        return fail(EXG_ERR_INVALID, "exg_world_create: need at least one body and one island");
+    if (!pose || !momentum || !mass || !inertia || !size || !island_start || !origin)
+        return fail(EXG_ERR_INVALID, "exg_world_create: null state array");
+    if (n_joints > 0 && (!kind || !a || !b || !params))
+        return fail(EXG_ERR_INVALID, "exg_world_create: null joint array");
+    if (island_start[0] != 0 || island_start[n_islands] != n_bodies)
+        return fail(EXG_ERR_INVALID, "exg_world_create: island_start must run from 0 to n_bodies");
+    // joints grouped by island, in declaration order, both ends in one island
+    std::vector<int> island_of(n_bodies);
+    int max_m = 0;
+    for (int i = 0; i < n_islands; ++i) {
+        int s = island_start[i], e = island_start[i + 1];
+        if (e <= s) return fail(EXG_ERR_INVALID, "exg_world_create: empty island " + std::to_string(i));
+        max_m = std::max(max_m, e - s);
+        for (int k = s; k < e; ++k) island_of[k] = i;
+    }
+    std::vector<int> jstart(n_islands + 1, 0);
+    int prev = 0;
+    for (int q = 0; q < n_joints; ++q) {
+        if (a[q] < 0 || a[q] >= n_bodies || b[q] < 0 || b[q] >= n_bodies || a[q] == b[q])
+            return fail(EXG_ERR_INVALID, "exg_world_create: joint " + std::to_string(q) + " has a bad body index");
+        int ia = island_of[a[q]];
+        if (island_of[b[q]] != ia)
+            return fail(EXG_ERR_INVALID, "exg_world_create: joint " + std::to_string(q) + " spans two islands");
+        if (ia < prev) return fail(EXG_ERR_INVALID, "exg_world_create: joints must be grouped island by island");
+        if (kind[q] < 0 || kind[q] > 3) return fail(EXG_ERR_NO_SHADER, "exg_world_create: unknown joint kind");
+        prev = ia;
+        jstart[ia + 1]++;
+    }
+    int max_j = 0;
+    for (int i = 0; i < n_islands; ++i) {
+        max_j = std::max(max_j, jstart[i + 1]);
+        jstart[i + 1] += jstart[i];
+    }
+    if (max_m > MAX_ISLAND_BODIES || max_j > MAX_ISLAND_JOINTS)
+        return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: islands up to 8 bodies and 16 joints are supported (got " +
+                                             std::to_string(max_m) + " bodies, " + std::to_string(max_j) + " joints)");
+
+    auto* w = new exg_world();
+    w->B = n_bodies;
+    w->I = n_islands;
+    w->J = n_joints;
+    w->G = G;
+    w->gravity = gravity_on != 0;
+    w->max_m = max_m;
+    w->max_j = max_j;
+    auto bail = [&](int code) { w->release(); delete w; return code; };
+#define W_TRY(expr)                                                                                    \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            fail(EXG_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+            return bail(EXG_ERR_BACKEND);                                                              \
+        }                                                                                              \
+    } while (0)
+    W_TRY(hipStreamCreate(&w->stream));
+    const size_t B = n_bodies, I = n_islands, Jn = n_joints;
+    W_TRY(w->pose.alloc(B * 8));
+    W_TRY(w->mom.alloc(B * 6));
+    W_TRY(w->mass.alloc(B));
+    W_TRY(w->tens.alloc(B * 9));
+    W_TRY(w->size.alloc(B));
+    W_TRY(w->ext.alloc(B * 6));
+    W_TRY(w->origin.alloc(I * 3));
+    W_TRY(w->charges[0].alloc(B * 8));
+    W_TRY(w->charges[1].alloc(B * 8));
+    W_TRY(w->island_start.alloc(I + 1));
+    W_TRY(w->island_jstart.alloc(I + 1));
+    W_TRY(w->seeded.alloc(I));
+    W_TRY(w->stats.alloc(I * exg::ISLAND_STATS));
+    W_TRY(w->xacc_off.alloc(I));
+    std::vector<long long> xoff(I);
+    long long tot = 0;
+    for (size_t i = 0; i < I; ++i) {
+        long long mm = island_start[i + 1] - island_start[i];
+        xoff[i] = tot;
+        tot += mm * mm * 36;
+    }
+    W_TRY(w->xacc.alloc((size_t)tot));
+    if (Jn) {
+        W_TRY(w->jkind.alloc(Jn));
+        W_TRY(w->ja.alloc(Jn));
+        W_TRY(w->jb.alloc(Jn));
+        W_TRY(w->jparams.alloc(Jn * 10));
+    }
+    auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
+    W_TRY(up(w->pose.p, pose, B * 8 * sizeof(double)));
+    W_TRY(up(w->mom.p, momentum, B * 6 * sizeof(double)));
+    W_TRY(up(w->mass.p, mass, B * sizeof(double)));
+    W_TRY(up(w->tens.p, inertia, B * 9 * sizeof(double)));
+    W_TRY(up(w->size.p, size, B * sizeof(double)));
+    W_TRY(up(w->origin.p, origin, I * 3 * sizeof(double)));
+    W_TRY(up(w->island_start.p, island_start, (I + 1) * sizeof(int)));
+    W_TRY(up(w->island_jstart.p, jstart.data(), (I + 1) * sizeof(int)));
+    W_TRY(up(w->xacc_off.p, xoff.data(), I * sizeof(long long)));
+    if (Jn) {
+        W_TRY(up(w->jkind.p, kind, Jn * sizeof(int)));
+        W_TRY(up(w->ja.p, a, Jn * sizeof(int)));
+        W_TRY(up(w->jb.p, b, Jn * sizeof(int)));
+        W_TRY(up(w->jparams.p, params, Jn * 10 * sizeof(double)));
+    }
+    // zero charges (com 0, G·m 0, size 1): nothing gravitates on the first step
+    std::vector<double> zc(B * 8, 0.0);
+    for (size_t i = 0; i < B; ++i) zc[i * 8 + 7] = 1.0;
+    W_TRY(up(w->charges[0].p, zc.data(), B * 8 * sizeof(double)));
+    W_TRY(up(w->charges[1].p, zc.data(), B * 8 * sizeof(double)));
+    W_TRY(hipMemset(w->xacc.p, 0, (size_t)tot * sizeof(double)));
+    W_TRY(hipMemset(w->seeded.p, 0, I * sizeof(int)));
+    W_TRY(hipMemset(w->stats.p, 0, I * exg::ISLAND_STATS * sizeof(int)));
+    W_TRY(hipMemset(w->ext.p, 0, B * 6 * sizeof(double)));
+#undef W_TRY
+    *out = w;
+    return EXG_OK;
+}
+
+int exg_world_destroy(exg_world* w) {
+    if (!w) return EXG_OK;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    w->release();
+    delete w;
+    return EXG_OK;
+}
+
+int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm, int ns_seed) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_solver: null world");
+    if (integrator != EXG_NEWTON) return fail(EXG_ERR_UNSUPPORTED, "exg_world_set_solver: only EXG_NEWTON runs on the device");
+    if (ns_warm < 0 || ns_seed < 1) return fail(EXG_ERR_INVALID, "exg_world_set_solver: bad Newton–Schulz budgets");
+    w->integrator = integrator;
+    w->tol2 = tol2;
+    w->ns_warm = ns_warm;
+    w->ns_seed = ns_seed;
+    return EXG_OK;
+}
+
+int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_step: null world");
+    if (n_steps < 0) return fail(EXG_ERR_INVALID, "exg_world_step: negative step count");
+    if (w->timing) {
+        size_t need = (size_t)n_steps * 3;
+        while (w->ev.size() < need) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            w->ev.push_back(e);
+        }
+    }
+    w->ev_used = 0;
+    for (int s = 0; s < n_steps; ++s) {
+        double* front = w->charges[w->front].p;
+        double* back = w->charges[1 - w->front].p;
+        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
+        if (w->gravity) {
+            exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->ext.p, w->stream);
+        } else {
+            HIP_TRY(hipMemsetAsync(w->ext.p, 0, (size_t)w->B * 6 * sizeof(double), w->stream));
+        }
+        HIP_TRY(hipGetLastError());
+        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 1], w->stream));
+        exg::IslandArgs a{};
+        a.pose = w->pose.p;
+        a.mom = w->mom.p;
+        a.mass = w->mass.p;
+        a.tens = w->tens.p;
+        a.size = w->size.p;
+        a.ext = w->ext.p;
+        a.island_start = w->island_start.p;
+        a.island_jstart = w->island_jstart.p;
+        a.origin = w->origin.p;
+        a.jkind = w->jkind.p;
+        a.ja = w->ja.p;
+        a.jb = w->jb.p;
+        a.jparams = w->jparams.p;
+        a.xacc = w->xacc.p;
+        a.xacc_off = w->xacc_off.p;
+        a.seeded = w->seeded.p;
+        a.charges_out = back;
+        a.stats = w->stats.p;
+        a.dt = dt;
+        a.warp = warp;
+        a.tol2 = w->tol2;
+        a.G = w->G;
+        a.ns_warm = w->ns_warm;
+        a.ns_seed = w->ns_seed;
+        a.n_islands = w->I;
+        exg::launch_island_step(a, w->max_m, w->max_j, w->stream);
+        HIP_TRY(hipGetLastError());
+        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
+        w->front = 1 - w->front;
+    }
+    w->ev_used = w->timing ? n_steps : 0;
+    return EXG_OK;
+}
+
+int exg_world_sync(exg_world* w) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_sync: null world");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return EXG_OK;
+}
+
+int exg_world_last_timing(exg_world* w, double* out2) {
+    if (!w || !out2) return fail(EXG_ERR_INVALID, "exg_world_last_timing: null argument");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    double g = 0.0, s = 0.0;
+    for (int k = 0; k < w->ev_used; ++k) {
+        float a = 0.f, b = 0.f;
+        HIP_TRY(hipEventElapsedTime(&a, w->ev[3 * k], w->ev[3 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, w->ev[3 * k + 1], w->ev[3 * k + 2]));
+        g += a;
+        s += b;
+    }
+    out2[0] = g;
+    out2[1] = s;
+    return EXG_OK;
+}
+
+int exg_world_download(exg_world* w, double* pose, double* momentum, double* origin) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_download: null world");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    if (pose) HIP_TRY(hipMemcpy(pose, w->pose.p, (size_t)w->B * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    if (momentum) HIP_TRY(hipMemcpy(momentum, w->mom.p, (size_t)w->B * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    if (origin) HIP_TRY(hipMemcpy(origin, w->origin.p, (size_t)w->I * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    return EXG_OK;
+}
+
+int exg_world_charges(exg_world* w, double* charges) {
+    if (!w || !charges) return fail(EXG_ERR_INVALID, "exg_world_charges: null argument");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    HIP_TRY(hipMemcpy(charges, w->charges[w->front].p, (size_t)w->B * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    return EXG_OK;
+}
+
+int exg_world_external(exg_world* w, double* ext) {
+    if (!w || !ext) return fail(EXG_ERR_INVALID, "exg_world_external: null argument");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    HIP_TRY(hipMemcpy(ext, w->ext.p, (size_t)w->B * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    return EXG_OK;
+}
+
+int exg_world_stats(exg_world* w, int32_t* stats) {
+    if (!w || !stats) return fail(EXG_ERR_INVALID, "exg_world_stats: null argument");
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    HIP_TRY(hipMemcpy(stats, w->stats.p, (size_t)w->I * exg::ISLAND_STATS * sizeof(int), hipMemcpyDeviceToHost));
+    return EXG_OK;
+}
+
+int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench) {
+    if (n <= 0 || !charges || !wrench) return fail(EXG_ERR_INVALID, "exg_gravity_accumulate: bad arguments");
+    double *dc = nullptr, *dw = nullptr;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dc), (size_t)n * 8 * sizeof(double)));
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dw), (size_t)n * 6 * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipFree(dc);
+        return fail(EXG_ERR_BACKEND, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpy(dc, charges, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        exg::launch_gravity_f64(dc, n, 1.0 / G, dw, nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(wrench, dw, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(dc);
+    (void)hipFree(dw);
+    if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_gravity_accumulate: ") + hipGetErrorString(e));
+    return EXG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// Launch interface of the device kernels of the dynamics hot path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace exg {
+
+constexpr int ISLAND_THREADS = 256;
+constexpr int ISLAND_STATS = 8;  // spans, newton iterations, NS iterations, reseeds, line-search rejects, unconverged commits
+
+// One implicit step of every island (one workgroup per island).
+struct IslandArgs {
+    double* pose;             // B×8 motors, island frame
+    double* mom;              // B×6 body-frame momenta
+    const double* mass;       // B
+    const double* tens;       // B×9 body-frame inertia tensor
+    const double* size;       // B effective size
+    const double* ext;        // B×6 external wrench, island frame
+    const int* island_start;  // I+1
+    const int* island_jstart; // I+1
+    double* origin;           // I×3
+    const int* jkind;         // J
+    const int* ja;            // J, global body index
+    const int* jb;            // J
+    const double* jparams;    // J×10
+    double* xacc;             // accepted approximate inverses, m²·36 per island
+    const long long* xacc_off;// I
+    int* seeded;              // I
+    double* charges_out;      // B×8: com xyz, local xyz, G·m, size
+    int* stats;               // I×ISLAND_STATS
+    double dt, warp, tol2, G;
+    int ns_warm, ns_seed, n_islands;
+};
+
+void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream);
+
+// Pairwise gravity: ext[i] = (Σ_j F_ij, local_i × Σ_j F_ij), charges B×8.
+void launch_gravity_f64(const double* charges, int n, double g_inv, double* ext, hipStream_t stream);
+
+}  // namespace exg

@@ -1,0 +1,677 @@
+// The implicit-midpoint Newton step of one island per workgroup, entirely on the
+// device: Jacobian evaluation, residual, block assembly, Newton–Schulz inverse,
+// the 8-lane line search, dt subdivision, commit, X checkpoint/rollback, and the
+// island re-anchoring + gravity-charge publication that close a Mechanism step.
+//
+// The control flow is the reference's host loop moved into the kernel:
+//   Newton::solve_step / newton_solve       crates/newton/src/integrator/implicit/newton.rs:334-683
+//   run_budget / refine_inverse / seed      newton.rs:209-327
+//   signal_log2 / commit_span               newton.rs:688-753
+//   baked-row arithmetic (a1·out + terms)   crates/newton/build.rs:1156-1403
+//   Island::recompute_centroid              crates/newton/src/mechanism/island.rs:206-244
+//   GravityPropagator::charge_of + publish  crates/newton/src/gravity.rs:485-496, 627-638
+//
+// Everything an island touches during the step lives in LDS (A, X, X', R, the
+// joint Jacobians, the 8 line-search lanes): one workgroup owns its island for
+// the whole step and HBM is touched only to load the state and write it back,
+// plus the accepted inverse X_acc on every commit/rollback. Every control
+// decision is taken redundantly by all threads from values broadcast through LDS,
+// so the workgroup stays converged.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "island.h"
+#include "physics_dev.h"
+
+namespace exg {
+
+constexpr int NT = ISLAND_THREADS;
+constexpr int LANES = 8;
+constexpr int MAX_ITERATIONS = 32;
+constexpr int MAX_DEPTH = 8;
+constexpr int THRESHOLD_LOG2 = 8;
+constexpr int PER_HALVING_LOG2 = 4;
+
+template <int MAXM, int MAXJ>
+struct IslandShared {
+    static constexpr int NB = MAXM * MAXM * 36;
+    double mat[4][NB];                      // A, X, X', R (X and X' swap by index)
+    double jac[MAXJ][2][24][6];             // ∂wrench[end]/∂axis, per joint
+    double vmid[LANES][MAXM][6];
+    double mid[LANES][MAXM][8];
+    double total[LANES][MAXM][6];
+    double rhs[LANES][MAXM][6];
+    double scale[LANES][MAXM][2];
+    double conn[LANES][MAXJ][2][6];
+    double wn[LANES];
+    double mass0[MAXM][36];                 // lane 0's world mass block
+    double pose[MAXM][8], mom[MAXM][6], massv[MAXM], tens[MAXM][9], ext[MAXM][6];
+    double snap[MAXM][6], dv[MAXM][6], base[MAXM][6], best[MAXM][6];
+    double jparams[MAXJ][10];
+    double red[NT];
+    double bcast[8];
+    int jkind[MAXJ], ja[MAXJ], jb[MAXJ];
+    int nbr_start[MAXM + 1], nbr[MAXM * MAXM];
+    int term_start[MAXM * MAXM + 1], terms[4 * MAXJ];
+    int inc_start[MAXM + 1], inc[2 * MAXJ];
+    int ibcast[8];
+};
+
+// Block-wide sum of one value per thread (tree order; the consumers compare the
+// result against a ×4 margin or take ⌊log₂⌋, never test it for equality).
+template <int MAXM, int MAXJ>
+__device__ double block_sum(IslandShared<MAXM, MAXJ>& s, double v) {
+    int t = threadIdx.x;
+    s.red[t] = v;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (t < w) s.red[t] = s.red[t] + s.red[t + w];
+        __syncthreads();
+    }
+    double r = s.red[0];
+    __syncthreads();
+    return r;
+}
+
+template <int MAXM, int MAXJ>
+struct IslandSolver {
+    IslandShared<MAXM, MAXJ>& s;
+    const IslandArgs& a;
+    int m, J, isl, first_body;
+    int ix, ixn;              // which of mat[1], mat[2] is X and which X'
+    double dt, warp, half, span;
+    long long xoff;
+    int st_spans, st_newton, st_ns, st_reseed, st_reject, st_uncommit;
+    bool seeded;
+
+    __device__ double* A() { return s.mat[0]; }
+    __device__ double* X() { return s.mat[ix]; }
+    __device__ double* Xn() { return s.mat[ixn]; }
+    __device__ double* R() { return s.mat[3]; }
+    static constexpr int blk(int m_, int i, int j) { return (i * m_ + j) * 36; }
+
+    // ───── PRE: midpoint = pose ∘ exp(half·vmid) for lanes [l0, l1) ─────
+    __device__ void pre(int l0, int l1) {
+        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NT) {
+            int lane = l0 + t / m, i = t % m;
+            double e[8];
+            twist_exp(s.vmid[lane][i], half, e);
+            compose(s.pose[i], e, s.mid[lane][i]);
+        }
+    }
+    // ───── Jacobian lanes: one thread per (joint, axis) ─────
+    __device__ void joints_jacobian() {
+        using D = Dual<double>;
+        for (int t = threadIdx.x; t < J * 24; t += NT) {
+            int j = t / 24, ax = t % 24;
+            int ends[2] = {s.ja[j], s.jb[j]};
+            D pp[2][8], vv[2][6];
+            for (int e = 0; e < 2; ++e) {
+                const double* base = s.mid[0][ends[e]];
+                if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
+                else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+                const double* v = s.vmid[0][ends[e]];
+                for (int k = 0; k < 6; ++k) vv[e][k] = D(v[k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
+            }
+            D wa[6], wb[6];
+            joint_wrench(s.jkind[j], s.jparams[j], pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+            for (int k = 0; k < 6; ++k) {
+                s.jac[j][0][ax][k] = wa[k].d;
+                s.jac[j][1][ax][k] = wb[k].d;
+            }
+            if (ax == 0)
+                for (int k = 0; k < 6; ++k) {
+                    s.conn[0][j][0][k] = wa[k].v;
+                    s.conn[0][j][1][k] = wb[k].v;
+                }
+        }
+    }
+    // ───── value-only joints for lanes [l0, l1) ─────
+    __device__ void joints_plain(int l0, int l1) {
+        for (int t = threadIdx.x; t < (l1 - l0) * J; t += NT) {
+            int lane = l0 + t / J, j = t % J;
+            joint_wrench(s.jkind[j], s.jparams[j], s.mid[lane][s.ja[j]], s.mid[lane][s.jb[j]], s.vmid[lane][s.ja[j]],
+                         s.vmid[lane][s.jb[j]], dt, warp, s.conn[lane][j][0], s.conn[lane][j][1]);
+        }
+    }
+    // ───── gather (external + incident wrenches in joint order) + body post ─────
+    __device__ void gather_post_one(int lane, int i, bool write_mass) {
+        double tot[6];
+        for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
+        for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
+            int code = s.inc[q];
+            const double* w = s.conn[lane][code >> 1][code & 1];
+            for (int k = 0; k < 6; ++k) tot[k] = tot[k] + w[k];
+        }
+        for (int k = 0; k < 6; ++k) s.total[lane][i][k] = tot[k];
+        double block[6][6];
+        const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
+        body_post(s.mid[lane][i], s.vmid[lane][i], s.massv[i], s.tens[i], s.snap[i], tot, half, floor2, block,
+                  s.rhs[lane][i], s.scale[lane][i]);
+        if (write_mass)
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = block[r][c];
+    }
+    __device__ void gather_post(int l0, int l1) {
+        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NT) {
+            int lane = l0 + t / m, i = t % m;
+            gather_post_one(lane, i, lane == 0);
+        }
+    }
+    static __device__ double max2(double x, double y) { return x > y ? x : y; }
+    // ───── worst² residual ratio per lane (newton.rs:139-151) ─────
+    __device__ void wnorm(int l0, int l1) {
+        for (int lane = l0 + (int)threadIdx.x; lane < l1; lane += NT) {
+            double worst = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double* r = s.rhs[lane][i];
+                double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+                double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
+                worst = max2(worst, lin2 / s.scale[lane][i][0]);
+                worst = max2(worst, ang2 / s.scale[lane][i][1]);
+            }
+            s.wn[lane] = worst;
+        }
+    }
+    // ───── system matrix: A = 0·A + [i==j]·𝕀 + Σ terms (pf·∂W/∂pose + vf·∂W/∂vel) ─────
+    __device__ void assemble() {
+        double pf = -(half * half), vf = -half;
+        double* Am = A();
+        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
+            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
+            double acc = 0.0 * Am[e] + ((i == j) ? 1.0 : 0.0) * s.mass0[i][rc];
+            for (int q = s.term_start[b]; q < s.term_start[b + 1]; ++q) {
+                int code = s.terms[q];
+                int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
+                double p = s.jac[jn][re][ce * 6 + c][r];
+                double v = s.jac[jn][re][12 + ce * 6 + c][r];
+                acc = acc + pf * p + vf * v;
+            }
+            Am[e] = acc;
+        }
+    }
+    // ───── out = 0·out + Σ_k a[i,k]·(sign·b[k,j] + diag·2I) ─────
+    __device__ void gemm(const double* am, const double* bm, double* out, bool two_minus, bool sparse) {
+        double sign = two_minus ? -1.0 : 1.0;
+        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
+            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
+            double acc = 0.0 * out[e];
+            int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : m;
+            for (int q = k0; q < k1; ++q) {
+                int k = sparse ? s.nbr[q] : q;
+                double diag = (two_minus && k == j) ? 1.0 : 0.0;
+                const double* ab = am + blk(m, i, k) + r * 6;
+                const double* bb = bm + blk(m, k, j);
+                double sum = 0.0;
+                for (int kk = 0; kk < 6; ++kk) {
+                    double right = sign * bb[kk * 6 + c];
+                    if (kk == c) right = right + diag * 2.0;
+                    sum = sum + ab[kk] * right;
+                }
+                acc = acc + sum;
+            }
+            out[e] = acc;
+        }
+    }
+    // ───── Σ ‖diag·I − b‖² over every block of `mat` (n_blocks blocks) ─────
+    __device__ double reduce_sq(const double* mat, int n_blocks, bool identity_diag) {
+        double part = 0.0;
+        for (int e = threadIdx.x; e < n_blocks * 36; e += NT) {
+            int b = e / 36, rc = e % 36, r = rc / 6, c = rc % 6;
+            double diag = (identity_diag && (b / m == b % m)) ? 1.0 : 0.0;
+            double v = mat[e];
+            double x = (r == c) ? diag - v : -v;
+            part = part + x * x;
+        }
+        return block_sum(s, part);
+    }
+    __device__ double refine_inverse() {  // newton.rs:255-279
+        gemm(A(), X(), R(), false, true);
+        __syncthreads();
+        double before = reduce_sq(R(), m * m, true);
+        gemm(X(), R(), Xn(), true, false);
+        __syncthreads();
+        int t = ix;
+        ix = ixn;
+        ixn = t;
+        st_ns++;
+        return before;
+    }
+    __device__ void seed_transpose() {  // newton.rs:209-244
+        // per block row i: Σ|A[i, ·]| and per block column i: Σ|A[·, i]|
+        if (threadIdx.x < 2 * m) {
+            int i = threadIdx.x % m;
+            bool col = threadIdx.x >= m;
+            double sum = 0.0;
+            for (int j = 0; j < m; ++j) {
+                const double* b = A() + (col ? blk(m, j, i) : blk(m, i, j));
+                for (int k = 0; k < 36; ++k) sum += fabs(b[k]);
+            }
+            s.red[threadIdx.x] = sum;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double one = 0.0, inf = 0.0;
+            for (int i = 0; i < m; ++i) {
+                inf = max2(inf, s.red[i]);
+                one = max2(one, s.red[m + i]);
+            }
+            s.bcast[0] = 1.0 / (one * inf);
+        }
+        __syncthreads();
+        double sc = s.bcast[0];
+        double* Xm = X();
+        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
+            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
+            Xm[e] = A()[blk(m, j, i) + c * 6 + r] * sc;
+        }
+        __syncthreads();
+        seeded = true;
+    }
+    __device__ void run_budget(int budget, int seed_budget) {  // newton.rs:301-327
+        bool has_prev = false, reseeded = false;
+        double previous = 0.0;
+        int remaining = budget;
+        while (remaining > 0) {
+            remaining -= 1;
+            double before = refine_inverse();
+            if (has_prev && !(before < 4.0 * previous) && !reseeded) {
+                seed_transpose();
+                reseeded = true;
+                has_prev = false;
+                remaining = seed_budget;
+                st_reseed++;
+                continue;
+            }
+            previous = before;
+            has_prev = true;
+        }
+    }
+    // ───── dv = 0·dv + Σ_j X[i,j]·rhs_j ─────
+    __device__ void matvec() {
+        const double* Xm = X();
+        for (int t = threadIdx.x; t < m * 6; t += NT) {
+            int i = t / 6, r = t % 6;
+            double acc = 0.0 * s.dv[i][r];
+            for (int j = 0; j < m; ++j) {
+                const double* xb = Xm + blk(m, i, j) + r * 6;
+                double sum = 0.0;
+                for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.rhs[0][j][c];
+                acc = acc + sum;
+            }
+            s.dv[i][r] = acc;
+        }
+    }
+    __device__ void copy_vmid(int dst, const double (*src)[6]) {
+        for (int t = threadIdx.x; t < m * 6; t += NT) s.vmid[dst][t / 6][t % 6] = src[t / 6][t % 6];
+    }
+
+    // ───── one Newton solve at the current `half` (newton.rs:334-530) ─────
+    __device__ void newton_solve(double tol2, int ns_warm, int ns_seed, bool& converged, double& best,
+                                 bool& assembled) {
+        const double huge = 1000000000.0;
+        // lane 0 iterate = world velocity of each body; the other lanes start from it
+        for (int t = threadIdx.x; t < m; t += NT)
+            world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.vmid[0][t]);
+        __syncthreads();
+        for (int t = threadIdx.x; t < (LANES - 1) * m * 6; t += NT) {
+            int lane = 1 + t / (m * 6), r = t % (m * 6);
+            s.vmid[lane][r / 6][r % 6] = s.vmid[0][r / 6][r % 6];
+        }
+        for (int t = threadIdx.x; t < m * 6; t += NT) s.best[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
+        __syncthreads();
+        best = huge;
+        converged = false;
+        assembled = false;
+        bool has_carried = false;
+        double carried = 0.0;
+        for (int iter = 0; iter < MAX_ITERATIONS; ++iter) {
+            double rnorm;
+            st_newton++;
+            if (has_carried) {
+                has_carried = false;
+                joints_jacobian();
+                __syncthreads();
+                rnorm = carried;
+            } else {
+                pre(0, 1);
+                __syncthreads();
+                joints_jacobian();
+                __syncthreads();
+                gather_post(0, 1);
+                __syncthreads();
+                wnorm(0, 1);
+                __syncthreads();
+                rnorm = s.wn[0];
+            }
+            if (rnorm < best) {
+                best = rnorm;
+                for (int t = threadIdx.x; t < m * 6; t += NT) s.best[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
+            }
+            if (rnorm <= tol2) { converged = true; break; }
+            if (!(rnorm < huge)) break;
+            assemble();
+            __syncthreads();
+            assembled = true;
+            int budget;
+            if (seeded) budget = ns_warm;
+            else { seed_transpose(); budget = ns_seed; }
+            run_budget(budget, ns_seed);
+            matvec();
+            __syncthreads();
+            // all-zero correction → hand over to subdivision
+            if (threadIdx.x == 0) {
+                int nz = 0;
+                for (int t = 0; t < m * 6; ++t) nz |= (s.dv[t / 6][t % 6] != 0.0);
+                s.ibcast[0] = nz;
+            }
+            __syncthreads();
+            if (!s.ibcast[0]) break;
+            // 8-lane backtracking line search from the same base
+            for (int t = threadIdx.x; t < m * 6; t += NT) s.base[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
+            __syncthreads();
+            for (int t = threadIdx.x; t < LANES * m * 6; t += NT) {
+                int lane = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
+                double alpha = 1.0;
+                for (int q = 0; q < lane; ++q) alpha = alpha / 2.0;
+                s.vmid[lane][i][k] = s.base[i][k] + s.dv[i][k] * alpha;
+            }
+            __syncthreads();
+            pre(0, LANES);
+            __syncthreads();
+            joints_plain(0, LANES);
+            __syncthreads();
+            gather_post(0, LANES);
+            __syncthreads();
+            wnorm(0, LANES);
+            __syncthreads();
+            int accepted = -1;
+            double tnorm = 0.0;
+            for (int l = 0; l < LANES; ++l)
+                if (s.wn[l] < rnorm) { accepted = l; tnorm = s.wn[l]; break; }
+            if (accepted >= 0) {
+                if (accepted > 0) {  // adopt_lane (cache.rs:1115-1136)
+                    for (int t = threadIdx.x; t < m * 6; t += NT) {
+                        s.vmid[0][t / 6][t % 6] = s.vmid[accepted][t / 6][t % 6];
+                        s.rhs[0][t / 6][t % 6] = s.rhs[accepted][t / 6][t % 6];
+                    }
+                    for (int t = threadIdx.x; t < m * 8; t += NT) s.mid[0][t / 8][t % 8] = s.mid[accepted][t / 8][t % 8];
+                    // the accepted lane's mass block, recomputed bit-identically from its inputs
+                    for (int t = threadIdx.x; t < m; t += NT) {
+                        double block[6][6], rr[6], sc[2];
+                        const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
+                        body_post(s.mid[accepted][t], s.vmid[accepted][t], s.massv[t], s.tens[t], s.snap[t],
+                                  s.total[accepted][t], half, floor2, block, rr, sc);
+                        for (int r = 0; r < 6; ++r)
+                            for (int c = 0; c < 6; ++c) s.mass0[t][r * 6 + c] = block[r][c];
+                    }
+                    __syncthreads();
+                }
+                has_carried = true;
+                carried = tnorm;
+            } else {
+                for (int t = threadIdx.x; t < m * 6; t += NT) s.vmid[0][t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
+                __syncthreads();
+                st_reject++;
+                break;
+            }
+        }
+        __syncthreads();
+        copy_vmid(0, s.best);
+        __syncthreads();
+    }
+
+    static __device__ bool log2_floor(double v, int& e) {  // newton.rs:101-121
+        double x = v;
+        e = 0;
+        if (!(x > 0.0)) return false;
+        while (x >= 2.0) {
+            x = x / 2.0;
+            e += 1;
+            if (e > 250) return false;
+        }
+        while (x < 1.0 && e > -250) {
+            x = x * 2.0;
+            e -= 1;
+        }
+        return true;
+    }
+    __device__ bool signal_log2(bool assembled, int& r) {
+        if (!assembled) return false;
+        double an = reduce_sq(A(), m * m, false);
+        double mn = 0.0;
+        {
+            double part = 0.0;
+            for (int e = threadIdx.x; e < m * 36; e += NT) {
+                double v = s.mass0[e / 36][e % 36];
+                part = part + v * v;
+            }
+            mn = block_sum(s, part);
+        }
+        double stiff = effective_zero(mn) ? 0.0 : an / mn;
+        return log2_floor(stiff, r);
+    }
+    __device__ void restore_x() {
+        double* Xm = X();
+        const double* src = a.xacc + xoff;
+        for (int e = threadIdx.x; e < m * m * 36; e += NT) Xm[e] = src[e];
+        __syncthreads();
+    }
+    __device__ void publish_x(bool& accepted_seeded) {
+        const double* Xm = X();
+        double* dst = a.xacc + xoff;
+        for (int e = threadIdx.x; e < m * m * 36; e += NT) dst[e] = Xm[e];
+        double n = reduce_sq(Xm, m * m, false);
+        const double huge = 1000000000.0;
+        if (n < huge * huge) accepted_seeded = seeded;
+        else { seeded = false; accepted_seeded = false; }
+    }
+    static __device__ int floor_div(int q, int d) { return (q >= 0) ? q / d : -((-q + d - 1) / d); }
+
+    // ───── solve_step (newton.rs:543-683) ─────
+    __device__ void solve_step(double tol2, int ns_warm, int ns_seed) {
+        bool accepted_seeded = seeded;
+        uint64_t remaining = 1ull << MAX_DEPTH;
+        int depth = 0;
+        while (remaining > 0) {
+            uint64_t units = 1ull << (MAX_DEPTH - depth);
+            while (units > remaining) { depth += 1; units >>= 1; }
+            span = dt;
+            for (int k = 0; k < depth; ++k) span = span / 2.0;
+            half = span / 2.0;
+            st_spans++;
+            for (int t = threadIdx.x; t < m; t += NT) world_momentum(s.pose[t], s.mom[t], s.snap[t]);
+            __syncthreads();
+            bool converged, assembled;
+            double best;
+            newton_solve(tol2, ns_warm, ns_seed, converged, best, assembled);
+            bool finite = best < 1000000000.0;
+            bool last = depth == MAX_DEPTH;
+            if (!converged && !last) {
+                restore_x();
+                seeded = accepted_seeded;
+                int r, down;
+                if (signal_log2(assembled, r)) down = floor_div(r - THRESHOLD_LOG2, PER_HALVING_LOG2) + 1;
+                else down = 1;
+                int hi = MAX_DEPTH - depth;
+                down = down < 1 ? 1 : (down > hi ? hi : down);
+                depth += down;
+                continue;
+            }
+            if (converged || finite) {
+                if (!converged) st_uncommit++;
+                for (int t = threadIdx.x; t < m; t += NT)
+                    commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.vmid[0][t], s.snap[t], span, half,
+                                converged);
+                __syncthreads();
+                publish_x(accepted_seeded);
+            } else {
+                restore_x();
+                seeded = accepted_seeded;
+            }
+            remaining -= units;
+            auto aligned = [&](int d) {
+                if (d <= 0) return false;
+                uint64_t bigger = 1ull << (MAX_DEPTH - (d - 1));
+                return bigger <= remaining && remaining % bigger == 0;
+            };
+            int r;
+            if (remaining > 0 && converged && aligned(depth) && signal_log2(assembled, r)) {
+                int up = floor_div(THRESHOLD_LOG2 - r, PER_HALVING_LOG2);
+                while (up > 0 && aligned(depth)) { depth -= 1; up -= 1; }
+            }
+        }
+    }
+};
+
+template <int MAXM, int MAXJ>
+__global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
+    __shared__ IslandShared<MAXM, MAXJ> s;
+    const int isl = blockIdx.x;
+    if (isl >= a.n_islands) return;
+    const int b0 = a.island_start[isl], m = a.island_start[isl + 1] - b0;
+    const int j0 = a.island_jstart[isl], J = a.island_jstart[isl + 1] - j0;
+    if (m > MAXM || J > MAXJ || m <= 0) return;  // the host routes islands by size; never taken
+
+    // ── load the island ──
+    for (int t = threadIdx.x; t < m * 8; t += NT) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
+    for (int t = threadIdx.x; t < m * 6; t += NT) {
+        s.mom[t / 6][t % 6] = a.mom[(size_t)(b0 + t / 6) * 6 + t % 6];
+        s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
+        s.dv[t / 6][t % 6] = 0.0;
+    }
+    for (int t = threadIdx.x; t < m * 9; t += NT) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];
+    for (int t = threadIdx.x; t < m; t += NT) s.massv[t] = a.mass[b0 + t];
+    for (int t = threadIdx.x; t < J; t += NT) {
+        s.jkind[t] = a.jkind[j0 + t];
+        s.ja[t] = a.ja[j0 + t] - b0;
+        s.jb[t] = a.jb[j0 + t] - b0;
+        for (int p = 0; p < 10; ++p) s.jparams[t][p] = a.jparams[(size_t)(j0 + t) * 10 + p];
+    }
+    const long long xoff = a.xacc_off[isl];
+    for (int e = threadIdx.x; e < m * m * 36; e += NT) {
+        s.mat[0][e] = 0.0;
+        s.mat[1][e] = a.xacc[xoff + e];
+        s.mat[2][e] = 0.0;
+        s.mat[3][e] = 0.0;
+    }
+    __syncthreads();
+    // ── topology tables: neighbours, assembly terms, incidence (joint order) ──
+    if (threadIdx.x == 0) {
+        int cnt[MAXM] = {};
+        for (int i = 0; i < m; ++i) { s.nbr[i * MAXM] = i; cnt[i] = 1; }
+        for (int j = 0; j < J; ++j) {
+            int ia = s.ja[j], ib = s.jb[j];
+            bool f = false;
+            for (int q = 0; q < cnt[ia]; ++q) f |= s.nbr[ia * MAXM + q] == ib;
+            if (!f) s.nbr[ia * MAXM + cnt[ia]++] = ib;
+            f = false;
+            for (int q = 0; q < cnt[ib]; ++q) f |= s.nbr[ib * MAXM + q] == ia;
+            if (!f) s.nbr[ib * MAXM + cnt[ib]++] = ia;
+        }
+        // compact the neighbour lists
+        int o = 0;
+        int tmp[MAXM * MAXM];
+        for (int i = 0; i < m; ++i) {
+            s.nbr_start[i] = o;
+            for (int q = 0; q < cnt[i]; ++q) tmp[o++] = s.nbr[i * MAXM + q];
+        }
+        s.nbr_start[m] = o;
+        for (int q = 0; q < o; ++q) s.nbr[q] = tmp[q];
+        o = 0;
+        for (int b = 0; b < m * m; ++b) {
+            s.term_start[b] = o;
+            int bi = b / m, bj = b % m;
+            for (int j = 0; j < J; ++j) {
+                int ends[2] = {s.ja[j], s.jb[j]};
+                for (int re = 0; re < 2; ++re)
+                    for (int ce = 0; ce < 2; ++ce)
+                        if (ends[re] == bi && ends[ce] == bj) s.terms[o++] = (j << 2) | (re << 1) | ce;
+            }
+        }
+        s.term_start[m * m] = o;
+        o = 0;
+        for (int i = 0; i < m; ++i) {
+            s.inc_start[i] = o;
+            for (int j = 0; j < J; ++j) {
+                if (s.ja[j] == i) s.inc[o++] = (j << 1) | 0;
+                if (s.jb[j] == i) s.inc[o++] = (j << 1) | 1;
+            }
+        }
+        s.inc_start[m] = o;
+    }
+    __syncthreads();
+
+    IslandSolver<MAXM, MAXJ> sv{s, a};
+    sv.m = m;
+    sv.J = J;
+    sv.isl = isl;
+    sv.first_body = b0;
+    sv.ix = 1;
+    sv.ixn = 2;
+    sv.dt = a.dt;
+    sv.warp = a.warp;
+    sv.xoff = xoff;
+    sv.st_spans = sv.st_newton = sv.st_ns = sv.st_reseed = sv.st_reject = sv.st_uncommit = 0;
+    sv.seeded = a.seeded[isl] != 0;
+    sv.solve_step(a.tol2, a.ns_warm, a.ns_seed);
+    __syncthreads();
+
+    // ── recompute_centroid (island.rs:206-244) and publish the charges ──
+    if (threadIdx.x == 0) {
+        double total_m = 0.0, wx[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < m; ++i) {
+            double p[3];
+            body_position(s.pose[i], p);
+            total_m += s.massv[i];
+            for (int k = 0; k < 3; ++k) wx[k] += p[k] * s.massv[i];
+        }
+        if (!effective_zero(total_m)) {
+            double inv = 1.0 / total_m;
+            double sh[6] = {-(wx[0] * inv), -(wx[1] * inv), -(wx[2] * inv), 0.0, 0.0, 0.0};
+            double shift[8];
+            twist_exp(sh, 1.0, shift);
+            for (int i = 0; i < m; ++i) {
+                double np[8];
+                compose(s.pose[i], shift, np);
+                for (int k = 0; k < 8; ++k) s.pose[i][k] = np[k];
+            }
+            for (int k = 0; k < 3; ++k) a.origin[isl * 3 + k] += wx[k] * inv;
+        }
+        for (int k = 0; k < 3; ++k) s.bcast[2 + k] = a.origin[isl * 3 + k];
+        s.ibcast[1] = sv.seeded ? 1 : 0;
+    }
+    __syncthreads();
+    double org[3] = {s.bcast[2], s.bcast[3], s.bcast[4]};
+    for (int t = threadIdx.x; t < m; t += NT) {
+        double p[3];
+        body_position(s.pose[t], p);
+        double* c = a.charges_out + (size_t)(b0 + t) * 8;
+        for (int k = 0; k < 3; ++k) {
+            c[k] = p[k] + org[k];
+            c[3 + k] = p[k];
+        }
+        c[6] = a.G * s.massv[t];
+        c[7] = a.size[b0 + t];
+    }
+    // ── write the state back ──
+    for (int t = threadIdx.x; t < m * 8; t += NT) a.pose[(size_t)(b0 + t / 8) * 8 + t % 8] = s.pose[t / 8][t % 8];
+    for (int t = threadIdx.x; t < m * 6; t += NT) a.mom[(size_t)(b0 + t / 6) * 6 + t % 6] = s.mom[t / 6][t % 6];
+    if (threadIdx.x == 0) {
+        a.seeded[isl] = s.ibcast[1];
+        int* st = a.stats + (size_t)isl * ISLAND_STATS;
+        st[0] += sv.st_spans;
+        st[1] += sv.st_newton;
+        st[2] += sv.st_ns;
+        st[3] += sv.st_reseed;
+        st[4] += sv.st_reject;
+        st[5] += sv.st_uncommit;
+    }
+}
+
+void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream) {
+    (void)max_m;
+    (void)max_j;
+    hipLaunchKernelGGL((island_step_kernel<8, 16>), dim3(a.n_islands), dim3(NT), 0, stream, a);
+}
+
+}  // namespace exg

@@ -1,0 +1,328 @@
+// Per-body and per-joint physics of the dynamics hot path, written once over the
+// scalar type S (double, float, Dual<double>, Dual<float>) for the HIP kernels.
+//
+// Semantics follow the reference's scalar-generic Rust bodies:
+//   Study-form seams        crates/peano/src/scalar.rs:67-135, thresholds support.rs:38-47
+//   Motor::exp / compose    crates/clifford/src/pga3/motor.rs:149-221
+//   screw transport         crates/clifford/src/pga3/screw.rs:86-101
+//   axial spring-damper     crates/joints/src/axial_spring_damper/mod.rs:315-385 (+ force laws)
+//   perpendicular damper    crates/joints/src/perpendicular_damper_warped.rs:107-162
+//   torsional damper        crates/joints/src/torsional_damper_warped.rs:93-117
+//   body_post               crates/functions/src/lib.rs:170-247
+//   commit_span             crates/newton/src/integrator/implicit/newton.rs:708-753
+// The geometric products come from pga3_gen.h (closed forms of the same
+// multivector expressions).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "dual.h"
+#include "pga3_gen.h"
+
+namespace exg {
+
+enum JointKind : int {
+    SIMPLE_SPRING_DAMPER = 0,
+    CRITICALLY_DAMPED_WARPED = 1,
+    PERPENDICULAR_DAMPER_WARPED = 2,
+    TORSIONAL_DAMPER_WARPED = 3,
+};
+
+// EffectiveZero: 1e-9 on f64, 1e-5 on f32 (support.rs:38-47)
+__host__ __device__ inline bool effective_zero(double x) { return fabs(x) < 1e-9; }
+__host__ __device__ inline bool effective_zero(float x) { return fabsf(x) < 1e-5f; }
+
+template <class S>
+__host__ __device__ inline S cst(double c) {
+    using R = typename RealOf<S>::type;
+    return S(R(c));
+}
+template <class S>
+__host__ __device__ inline S ratio(int num, int den) {  // from_rational: num as R / den as R
+    using R = typename RealOf<S>::type;
+    return S(R(num) / R(den));
+}
+
+template <class S>
+__host__ __device__ inline S sinc_sq(const S& u) {
+    if (effective_zero(value_of(u))) {
+        S c1 = ratio<S>(1, 6), c2 = ratio<S>(1, 120), c3 = ratio<S>(1, 5040);
+        return cst<S>(1.0) - u * (c1 - u * (c2 - u * c3));
+    }
+    S l = s_sqrt(u);
+    return s_sin(l) / l;
+}
+template <class S>
+__host__ __device__ inline S cos_sq(const S& u) {
+    if (effective_zero(value_of(u))) {
+        S c1 = ratio<S>(1, 2), c2 = ratio<S>(1, 24), c3 = ratio<S>(1, 720);
+        return cst<S>(1.0) - u * (c1 - u * (c2 - u * c3));
+    }
+    return s_cos(s_sqrt(u));
+}
+template <class S>
+__host__ __device__ inline S dsinc_sq(const S& u) {
+    if (effective_zero(value_of(u * ratio<S>(1, 100000)))) {
+        S c0 = ratio<S>(1, 6), c1 = ratio<S>(1, 60), c2 = ratio<S>(1, 1680), c3 = ratio<S>(1, 90720);
+        return -c0 + u * (c1 - u * (c2 - u * c3));
+    }
+    S l = s_sqrt(u);
+    return (l * s_cos(l) - s_sin(l)) / (cst<S>(2.0) * l * l * l);
+}
+
+// Motor::exp of the bivector with screw coordinates t·k (motor.rs:194-221).
+template <class S>
+__host__ __device__ inline void motor_exp_scaled(const S t[6], const S& k, S m[8]) {
+    S tt[6];
+    for (int i = 0; i < 6; ++i) tt[i] = t[i] * k;
+    S u, p, bv[8], ibv[8];
+    exp_parts(tt, u, p, bv, ibv);
+    S cu = cos_sq(u), su = sinc_sq(u), du = dsinc_sq(u);
+    S pdu = p * du;
+    m[0] = cu;
+    for (int i = 1; i < 7; ++i) m[i] = bv[i] * su - ibv[i] * pdu;
+    m[7] = p * ratio<S>(1, 2) * su;
+}
+// Twist::exp(dt) = Motor::exp(T·(dt·½)) (screw.rs:254-260)
+template <class S>
+__host__ __device__ inline void twist_exp(const S t[6], const S& dt, S m[8]) {
+    motor_exp_scaled(t, dt * ratio<S>(1, 2), m);
+}
+// self.compose(rhs) = rhs·self (motor.rs:149-158)
+template <class S>
+__host__ __device__ inline void compose(const S self[8], const S rhs[8], S out[8]) { motor_mul(rhs, self, out); }
+
+template <class S>
+__host__ __device__ inline void mat6_vec(const S a[6][6], const S x[6], S y[6]) {
+    for (int r = 0; r < 6; ++r) {
+        S acc = a[r][0] * x[0];
+        for (int c = 1; c < 6; ++c) acc = acc + a[r][c] * x[c];
+        y[r] = acc;
+    }
+}
+// Contra (twist) and Co (wrench) transport under a motor.
+template <class S>
+__host__ __device__ inline void transport_contra(const S t[6], const S m[8], S out[6]) {
+    S ad[6][6];
+    ad_matrix(m, ad);
+    mat6_vec(ad, t, out);
+}
+template <class S>
+__host__ __device__ inline void transport_co(const S w[6], const S m[8], S out[6]) {
+    S cad[6][6];
+    coad_matrix(m, cad);
+    mat6_vec(cad, w, out);
+}
+
+// ───────────────────────── inertia (inertia.rs:11-47, 292-333) ─────────────────────────
+template <class S>
+__host__ __device__ inline void inertia_apply(const S& mass, const S t9[9], const S v[6], S p[6]) {
+    for (int i = 0; i < 3; ++i) p[i] = v[i] * mass;
+    for (int r = 0; r < 3; ++r) p[3 + r] = t9[3 * r] * v[3] + t9[3 * r + 1] * v[4] + t9[3 * r + 2] * v[5];
+}
+template <class S>
+__host__ __device__ inline void inertia_apply_inverse(const S& mass, const S a[9], const S p[6], S v[6]) {
+    S inv = cst<S>(1.0) / mass;
+    for (int i = 0; i < 3; ++i) v[i] = p[i] * inv;
+    // solve3 by cofactors (inertia.rs:23-47)
+    S c00 = a[4] * a[8] - a[5] * a[7];
+    S c01 = a[5] * a[6] - a[3] * a[8];
+    S c02 = a[3] * a[7] - a[4] * a[6];
+    S c10 = a[2] * a[7] - a[1] * a[8];
+    S c11 = a[0] * a[8] - a[2] * a[6];
+    S c12 = a[1] * a[6] - a[0] * a[7];
+    S c20 = a[1] * a[5] - a[2] * a[4];
+    S c21 = a[2] * a[3] - a[0] * a[5];
+    S c22 = a[0] * a[4] - a[1] * a[3];
+    S det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    const S* b = p + 3;
+    v[3] = (c00 * b[0] + c10 * b[1] + c20 * b[2]) / det;
+    v[4] = (c01 * b[0] + c11 * b[1] + c21 * b[2]) / det;
+    v[5] = (c02 * b[0] + c12 * b[1] + c22 * b[2]) / det;
+}
+
+// World twist Ad_M(I⁻¹P) and world momentum Ad*_M P (body.rs:145-197).
+template <class S>
+__host__ __device__ inline void world_velocity(const S pose[8], const S& mass, const S t9[9], const S mom[6], S out[6]) {
+    S vb[6];
+    inertia_apply_inverse(mass, t9, mom, vb);
+    transport_contra(vb, pose, out);
+}
+template <class S>
+__host__ __device__ inline void world_momentum(const S pose[8], const S mom[6], S out[6]) {
+    transport_co(mom, pose, out);
+}
+// Position of the body origin: (M·O·M̃).coords()
+template <class S>
+__host__ __device__ inline void body_position(const S pose[8], S pos[3]) {
+    S z[3] = {cst<S>(0.0), cst<S>(0.0), cst<S>(0.0)};
+    S w, q[3];
+    point_sandwich(pose, z, w, q);
+    S winv = cst<S>(1.0) / w;
+    for (int i = 0; i < 3; ++i) pos[i] = q[i] * winv;
+}
+
+// ───────────────────────── joints ─────────────────────────
+// Wrenches [on a, on b] at world poses and world velocities (a pure world function).
+// `params` in the canonical order of each joint (joints/src/lib.rs:273-299).
+template <class S>
+__host__ __device__ inline void joint_wrench(int kind, const double* params, const S pa_pose[8], const S pb_pose[8],
+                                             const S va[6], const S vb[6], double dt, double warp, S wa_out[6],
+                                             S wb_out[6]) {
+    using R = typename RealOf<S>::type;
+    S vrel[6];
+    for (int i = 0; i < 6; ++i) vrel[i] = va[i] - vb[i];
+    if (kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED) {
+        S la[3] = {S(R(params[0])), S(R(params[1])), S(R(params[2]))};
+        S lb[3] = {S(R(params[3])), S(R(params[4])), S(R(params[5]))};
+        S wa, qa[3], wb, qb[3];
+        point_sandwich(pa_pose, la, wa, qa);
+        point_sandwich(pb_pose, lb, wb, qb);
+        S l[6];
+        join_points(wa, qa, wb, qb, l);
+        S eps = S(R(params[9]));
+        S d = s_sqrt(line_norm2(l) + eps * eps);
+        S raw[6];
+        wrench_from_line(l, raw);
+        S inv_d = cst<S>(1.0) / d;
+        S unit[6];
+        for (int i = 0; i < 6; ++i) unit[i] = raw[i] * inv_d;
+        S d_dot = pairing(unit, vrel);
+        S rest = S(R(params[6])), k = S(R(params[7])), c = S(R(params[8]));
+        S f;
+        if (kind == SIMPLE_SPRING_DAMPER) {
+            f = k * (d - rest) + c * d_dot;
+        } else {
+            S wp = S(R(warp));
+            S c0 = k / (wp * wp);
+            S omega_0 = s_sqrt(c0);
+            S dts = S(R(dt));
+            S den = cst<S>(1.0) + omega_0 * dts;
+            f = (c0 * (d - rest) + (cst<S>(1.0) + cst<S>(1.0)) * omega_0 * d_dot + c0 * dts * d_dot) / (den * den);
+        }
+        S nf = -f;
+        for (int i = 0; i < 6; ++i) {
+            wa_out[i] = unit[i] * nf;
+            wb_out[i] = -wa_out[i];
+        }
+    } else if (kind == PERPENDICULAR_DAMPER_WARPED) {
+        S z[3] = {cst<S>(0.0), cst<S>(0.0), cst<S>(0.0)};
+        S wa, qa[3], wb, qb[3];
+        point_sandwich(pa_pose, z, wa, qa);
+        point_sandwich(pb_pose, z, wb, qb);
+        S l[6];
+        join_points(wa, qa, wb, qb, l);
+        S eps = S(R(params[1]));
+        S d = s_sqrt(line_norm2(l) + eps * eps);
+        S wp = S(R(warp));
+        S b_eff = S(R(params[0])) / (wp * wp);
+        S dir[3];
+        line_direction(l, dir);
+        S inv_d = cst<S>(1.0) / d;
+        S n[3] = {dir[0] * inv_d, dir[1] * inv_d, dir[2] * inv_d};
+        S v_rel[3];
+        velocity_at(vrel, wa, qa, v_rel);
+        S v_dot_n = v_rel[0] * n[0] + (v_rel[1] * n[1] + (v_rel[2] * n[2] + cst<S>(0.0)));
+        S nb = -b_eff;
+        S winv = cst<S>(1.0) / wa;
+        S tip[3];
+        for (int i = 0; i < 3; ++i) {
+            S v_perp = v_rel[i] - n[i] * v_dot_n;
+            S f = v_perp * nb;
+            tip[i] = qa[i] * winv + f;
+        }
+        S l2[6];
+        join_points(cst<S>(1.0), tip, wa, qa, l2);
+        wrench_from_line(l2, wa_out);
+        for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
+    } else {
+        S wp = S(R(warp));
+        S b_eff = S(R(params[0])) / (wp * wp);
+        S nb = -b_eff;
+        for (int i = 0; i < 3; ++i) {
+            wa_out[i] = cst<S>(0.0);
+            wa_out[3 + i] = vrel[3 + i] * nb;
+        }
+        for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
+    }
+}
+
+// Derivative seeds of axis `axis` (0..23) for the unit retraction base∘exp(δ):
+// exp(δ) = 1 + δ at δ = 0 with a unit derivative, so the seeded motor is base with
+// derivative gp(B_k, base), B_k the screw basis of the pose axis (ad.rs:119-150).
+template <class T>
+__host__ __device__ inline void seed_pose(const T base[8], int dof, Dual<T> out[8]) {
+    T t6[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    t6[dof] = T(1);
+    T u, p, bv[8], ibv[8];
+    exp_parts(t6, u, p, bv, ibv);  // bv = the basis bivector in even-store layout
+    T d[8];
+    motor_mul(bv, base, d);
+    for (int i = 0; i < 8; ++i) out[i] = Dual<T>(base[i], d[i]);
+}
+
+// ───────────────────────── body_post (functions/src/lib.rs:170-247) ─────────────────────────
+// World mass block Ad*_M ∘ I ∘ Ad_{M⁻¹}, the residual −(𝕀V − P_n − half·W) and its
+// self-scale (lin², ang²).
+template <class S>
+__host__ __device__ inline void body_post(const S pose_mid[8], const S vel[6], const S& mass, const S t9[9],
+                                          const S snap[6], const S total[6], const S& half, const S& floor2,
+                                          S block[6][6], S rhs[6], S scale2[2]) {
+    S inv[8];
+    motor_rev(pose_mid, inv);
+    S adi[6][6], cad[6][6];
+    ad_matrix(inv, adi);
+    coad_matrix(pose_mid, cad);
+    for (int col = 0; col < 6; ++col) {
+        S x[6];
+        for (int i = 0; i < 3; ++i) x[i] = adi[i][col] * mass;
+        for (int r = 0; r < 3; ++r)
+            x[3 + r] = t9[3 * r] * adi[3][col] + t9[3 * r + 1] * adi[4][col] + t9[3 * r + 2] * adi[5][col];
+        for (int r = 0; r < 6; ++r) {
+            S acc = cad[r][0] * x[0];
+            for (int k = 1; k < 6; ++k) acc = acc + cad[r][k] * x[k];
+            block[r][col] = acc;
+        }
+    }
+    S p[6], hh[6];
+    for (int r = 0; r < 6; ++r) {
+        S acc = cst<S>(0.0);
+        for (int c = 0; c < 6; ++c) acc = acc + block[r][c] * vel[c];
+        p[r] = acc;
+        hh[r] = total[r] * half;
+        rhs[r] = -(p[r] - snap[r] - hh[r]);
+    }
+    for (int o = 0; o < 2; ++o) {
+        int b = 3 * o;
+        S sp_ = p[b] * p[b] + p[b + 1] * p[b + 1] + p[b + 2] * p[b + 2];
+        S sn = snap[b] * snap[b] + snap[b + 1] * snap[b + 1] + snap[b + 2] * snap[b + 2];
+        S sh = hh[b] * hh[b] + hh[b + 1] * hh[b + 1] + hh[b + 2] * hh[b + 2];
+        scale2[o] = fmax(fmax(fmax(sp_, sn), sh), floor2);
+    }
+}
+
+// commit_span for one body (newton.rs:708-753): pose for the full span by left
+// composition, momentum by the midpoint rule (2·P_mid − P_n when converged, else
+// the implicit-Euler reading P_mid), back to the body frame.
+template <class S>
+__host__ __device__ inline void commit_body(S pose[8], S mom[6], const S& mass, const S t9[9], const S vs[6],
+                                            const S snap[6], const S& span, const S& half, bool converged) {
+    S e[8], pose_mid[8], inv[8];
+    twist_exp(vs, half, e);
+    compose(pose, e, pose_mid);
+    motor_rev(pose_mid, inv);
+    S v_body[6], pb[6], p_mid[6];
+    transport_contra(vs, inv, v_body);
+    inertia_apply(mass, t9, v_body, pb);
+    transport_co(pb, pose_mid, p_mid);
+    S p_new[6];
+    for (int i = 0; i < 6; ++i) p_new[i] = converged ? p_mid[i] * cst<S>(2.0) - snap[i] : p_mid[i];
+    S e2[8], m_new[8], inv2[8];
+    twist_exp(vs, span, e2);
+    compose(pose, e2, m_new);
+    motor_rev(m_new, inv2);
+    S mb[6];
+    transport_co(p_new, inv2, mb);
+    for (int i = 0; i < 8; ++i) pose[i] = m_new[i];
+    for (int i = 0; i < 6; ++i) mom[i] = mb[i];
+}
+
+}  // namespace exg

@@ -1,0 +1,115 @@
+"""Device-resident world: the host-side mirror of the reference's stepping API.
+
+`DeviceWorld.step(n, dt, warp)` is one `Mechanism::step` per step over every island
+followed by the gravity propagator's `advance_epoch`
+(crates/newton/src/mechanism/mechanism.rs:526-600, crates/newton/src/gravity.rs:392-397),
+with the state resident in HBM between calls. Scenes come from exgame_amd.scenes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, pga
+
+INTEGRATORS = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2}
+
+
+@dataclass(frozen=True)
+class Epoch:
+    """dt and time warp of one step (crates/aristotle/src/epoch.rs:238-248)."""
+    dt: float
+    warp: float = 1.0
+
+
+class DeviceWorld:
+    def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
+                 ns_warm: int = 2, ns_seed: int = 24):
+        L = _lib.lib()
+        s = scene
+        self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
+        self.island_start = np.ascontiguousarray(s.island_start, dtype=np.int32)
+        arrs = dict(
+            pose=np.ascontiguousarray(s.pose, dtype=np.float64),
+            mom=np.ascontiguousarray(s.momentum, dtype=np.float64),
+            mass=np.ascontiguousarray(s.mass, dtype=np.float64),
+            tens=np.ascontiguousarray(s.inertia, dtype=np.float64),
+            size=np.ascontiguousarray(s.size, dtype=np.float64),
+            origin=np.ascontiguousarray(s.origin, dtype=np.float64),
+            jp=np.ascontiguousarray(s.joint_params, dtype=np.float64).reshape(-1),
+        )
+        ints = dict(
+            jk=np.ascontiguousarray(s.joint_kind, dtype=np.int32),
+            ja=np.ascontiguousarray(s.joint_a, dtype=np.int32),
+            jb=np.ascontiguousarray(s.joint_b, dtype=np.int32),
+        )
+        if self.nj == 0:
+            arrs["jp"] = np.zeros(10)
+            for k in ints:
+                ints[k] = np.zeros(1, dtype=np.int32)
+        h = C.c_void_p()
+        _lib.check(L.exg_world_create(
+            self.nb, _lib.fptr(arrs["pose"]), _lib.fptr(arrs["mom"]), _lib.fptr(arrs["mass"]), _lib.fptr(arrs["tens"]),
+            _lib.fptr(arrs["size"]), self.ni, _lib.iptr(self.island_start), _lib.fptr(arrs["origin"]), self.nj,
+            _lib.iptr(ints["jk"]), _lib.iptr(ints["ja"]), _lib.iptr(ints["jb"]), _lib.fptr(arrs["jp"]), float(G),
+            int(bool(gravity)), C.byref(h)))
+        self.h = h
+        _lib.check(L.exg_world_set_solver(self.h, INTEGRATORS[integrator], float(tol2), int(ns_warm), int(ns_seed)))
+
+    def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
+        if isinstance(dt, Epoch):
+            dt, warp = dt.dt, dt.warp
+        _lib.check(_lib.lib().exg_world_step(self.h, int(n), float(dt), float(warp)))
+
+    def sync(self) -> None:
+        _lib.check(_lib.lib().exg_world_sync(self.h))
+
+    def state(self) -> dict:
+        pose = np.zeros((self.nb, 8))
+        mom = np.zeros((self.nb, 6))
+        origin = np.zeros((self.ni, 3))
+        _lib.check(_lib.lib().exg_world_download(self.h, _lib.fptr(pose), _lib.fptr(mom), _lib.fptr(origin)))
+        isl = np.repeat(np.arange(self.ni), np.diff(self.island_start))
+        position = pga.position(pose) + origin[isl]
+        return {"pose": pose, "momentum": mom, "origin": origin, "position": position}
+
+    def charges(self) -> np.ndarray:
+        out = np.zeros((self.nb, 8))
+        _lib.check(_lib.lib().exg_world_charges(self.h, _lib.fptr(out)))
+        return out
+
+    def external(self) -> np.ndarray:
+        out = np.zeros((self.nb, 6))
+        _lib.check(_lib.lib().exg_world_external(self.h, _lib.fptr(out)))
+        return out
+
+    def stats(self) -> np.ndarray:
+        out = np.zeros((self.ni, 8), dtype=np.int32)
+        _lib.check(_lib.lib().exg_world_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
+        return out
+
+    def last_timing(self) -> dict:
+        out = np.zeros(2)
+        _lib.check(_lib.lib().exg_world_last_timing(self.h, _lib.fptr(out)))
+        return {"gravity_ms": float(out[0]), "island_ms": float(out[1])}
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            _lib.lib().exg_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gravity_accumulate(charges: np.ndarray, G: float = 1.0) -> np.ndarray:
+    ch = np.ascontiguousarray(charges, dtype=np.float64)
+    n = ch.size // 8
+    out = np.zeros((n, 6))
+    _lib.check(_lib.lib().exg_gravity_accumulate(n, _lib.fptr(ch), float(G), _lib.fptr(out)))
+    return out

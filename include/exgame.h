@@ -1,0 +1,134 @@
+/*
+ * exgame.h — C ABI of the MI355X-native per-step dynamics library (libexgame_hip.so).
+ *
+ * The library replaces the device half of the reference's accelerator boundary:
+ * what `newton::Accelerator<T>` dispatches (crates/newton/src/accelerator/mod.rs:557-965)
+ * on the `rembrandt` Vulkan device layer (crates/rembrandt/src/lib.rs:77-243) over
+ * `aristotle::World` storage (crates/aristotle/src/world.rs:395-660). Every entry
+ * point is plain C: pointers, sizes and integer status codes; no torch or HIP types.
+ *
+ * Data conventions (shared by every entry point):
+ *   motor  8 doubles — even blades ascending {1, e12, e13, e23, e01, e02, e03, e0123}
+ *          (crates/clifford/src/pga3/motor.rs:119-152, to_components/from_components)
+ *   screw  6 doubles — (linear|force x y z, angular|torque x y z), the coordinate
+ *          order of Twist::new / Wrench::new (crates/clifford/src/pga3/screw.rs:235-290)
+ *   charge 8 doubles — com xyz (absolute), local xyz (arm about the island origin),
+ *          G·m, effective size (crates/newton/src/gravity.rs:67-76)
+ *   joint params 10 doubles — the joint's canonical order (crates/joints/src/lib.rs:273-299):
+ *          axial [a0 a1 a2 b0 b1 b2 rest k c eps], perpendicular [c eps], torsional [c]
+ *
+ * Status codes map 1:1 onto the reference's EvalError (crates/newton/src/error.rs:11-43)
+ * plus two for argument and capability errors; exg_last_error() describes the last one.
+ * One world is driven from one host thread at a time.
+ */
+#ifndef EXGAME_H
+#define EXGAME_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum exg_status {
+    EXG_OK = 0,
+    EXG_ERR_NO_SHADER = 1,    /* EvalError::NoShader — no kernel for a joint kind / message kind */
+    EXG_ERR_TRAP = 2,         /* EvalError::Trap — a kernel trapped */
+    EXG_ERR_FATAL = 3,        /* EvalError::Fatal — guarded fatal slots fired */
+    EXG_ERR_WORKER_GONE = 4,  /* EvalError::WorkerGone — the device context is gone */
+    EXG_ERR_BACKEND = 5,      /* EvalError::Backend — a HIP runtime call failed */
+    EXG_ERR_INVALID = 6,      /* bad arguments (sizes, indices, topology) */
+    EXG_ERR_UNSUPPORTED = 7   /* a shape this build does not handle (e.g. island too large) */
+};
+
+/* Joint kinds — the four force laws of crates/joints (MessageKind families,
+ * crates/newton/src/accelerator/mod.rs:55-77). */
+enum exg_joint_kind {
+    EXG_SIMPLE_SPRING_DAMPER = 0,
+    EXG_CRITICALLY_DAMPED_WARPED = 1,
+    EXG_PERPENDICULAR_DAMPER_WARPED = 2,
+    EXG_TORSIONAL_DAMPER_WARPED = 3
+};
+
+/* Integrators of a world (crates/newton/src/integrator/implicit/mod.rs:20-60). */
+enum exg_integrator {
+    EXG_EXPLICIT_EULER = 0,
+    EXG_SYMPLECTIC_EULER = 1,
+    EXG_NEWTON = 2
+};
+
+/* Device description — the analogue of rembrandt's GpuAccelerator::new + gpu_in_flight
+ * (crates/rembrandt/src/lib.rs:77-243). */
+typedef struct exg_device_info {
+    int compute_units;
+    int wave_size;
+    int64_t lds_bytes_per_cu;
+    int64_t hbm_bytes;
+    char name[64];
+} exg_device_info;
+
+/* Last error message of the calling thread (empty string if none). */
+const char* exg_last_error(void);
+
+/* Fills `out` for device 0. */
+int exg_device_info_get(exg_device_info* out);
+
+typedef struct exg_world exg_world;
+
+/* A world of islands of rigid bodies connected by joints, plus N-body gravity —
+ * the device-resident state a `Mechanism` (crates/newton/src/mechanism/mechanism.rs:526-600)
+ * steps, laid out structure-of-arrays in HBM.
+ *
+ *   n_bodies B: pose[B*8], momentum[B*6] (body frame), mass[B], inertia[B*9] (body-frame
+ *               tensor, row-major), size[B] (effective size, crates/newton/src/body.rs:70-77)
+ *   n_islands I: island_start[I+1] (bodies of island i are [start[i], start[i+1])),
+ *               origin[I*3] (island anchor, absolute frame)
+ *   n_joints J: kind[J], a[J], b[J] (global body indices inside one island, joints
+ *               grouped island by island in declaration order), params[J*10]
+ *   G: gravitational constant (gravity off when gravity_on == 0)
+ * Returns EXG_ERR_UNSUPPORTED for an island beyond this build's on-chip size limits. */
+int exg_world_create(int n_bodies, const double* pose, const double* momentum, const double* mass,
+                     const double* inertia, const double* size, int n_islands, const int* island_start,
+                     const double* origin, int n_joints, const int* kind, const int* a, const int* b,
+                     const double* params, double G, int gravity_on, exg_world** out);
+int exg_world_destroy(exg_world* w);
+
+/* Solver settings: integrator, Newton tolerance² on the worst residual ratio
+ * (newton.rs:139-151, 802-805), Newton–Schulz warm/cold budgets (newton.rs:58-61). */
+int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm, int ns_seed);
+
+/* Advance `n_steps` steps of length dt with time warp `warp` (aristotle Epoch dt/warp,
+ * crates/aristotle/src/epoch.rs:238-248): per step, gravity from the charges published
+ * by the previous step (zero on the first, gravity.rs:620-626), the island solve, the
+ * re-anchoring and the new charges. Asynchronous on the world's stream. */
+int exg_world_step(exg_world* w, int n_steps, double dt, double warp);
+
+/* Block until the world's queued work has finished. */
+int exg_world_sync(exg_world* w);
+
+/* Copy the state back: pose[B*8], momentum[B*6], origin[I*3]; any pointer may be NULL. */
+int exg_world_download(exg_world* w, double* pose, double* momentum, double* origin);
+
+/* Charges the next step's gravity will read, B*8. */
+int exg_world_charges(exg_world* w, double* charges);
+
+/* External wrench each body received in the last step (gravity), B*6, island frame. */
+int exg_world_external(exg_world* w, double* ext);
+
+/* Per-island counters since creation, I*8: spans, Newton iterations, Newton–Schulz
+ * iterations, reseeds, line-search rejections, unconverged commits, 0, 0. */
+int exg_world_stats(exg_world* w, int32_t* stats);
+
+/* Device time of the last exg_world_step call, split by kernel, in milliseconds:
+ * out[0] gravity, out[1] island solve (sums over the steps of that call). */
+int exg_world_last_timing(exg_world* w, double* out2);
+
+/* Stand-alone pairwise gravity on the device (gravity.rs:110-146, 530-601):
+ * charges[n*8] in, wrench[n*6] out (force, local × force), host pointers. */
+int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EXGAME_H */
