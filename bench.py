@@ -1,0 +1,161 @@
+"""Throughput of the per-step dynamics hot path on one MI355X.
+
+Workload (the plain implicit Newton configuration, f64): 4,096 rigid bodies in 512
+chains of 8 (each chain one island), adjacent bodies joined by an axial
+spring-damper (k = 1e3, c = 10) and a torsional damper (c = 5), N-body gravity
+(G = 1, size-core law) on, h = 1e-2, Newton tolerance 1e-10 on the worst per-block
+residual ratio, Newton–Schulz 2 warm / 24 cold, a 200-step rollout.
+
+A bench step is one world step: gravity over all bodies, then every island's
+implicit Newton solve, re-anchoring and charge publication. The timed region runs
+K steps of a freshly uploaded world (state resident in HBM before the clock
+starts), bracketed by device synchronisation. One JSON line on stdout.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+CHAINS, M, H, TOL = 512, 8, 1e-2, 1e-10
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def ns_flops_per_iteration(scene) -> np.ndarray:
+    """Algorithmic flops of one Newton–Schulz iteration per island: the sparse
+    R = A·X (2·36·Σ_i |nbr(i)|·m per block column) plus the dense X·(2I − R)
+    (2·(6m)³)."""
+    out = []
+    for i in range(scene.n_islands):
+        s, e = scene.island_start[i], scene.island_start[i + 1]
+        m = e - s
+        js, je = scene.island_joint_start[i], scene.island_joint_start[i + 1]
+        nbr = [{k} for k in range(m)]
+        for q in range(js, je):
+            a, b = scene.joint_a[q] - s, scene.joint_b[q] - s
+            nbr[a].add(b)
+            nbr[b].add(a)
+        sparse = 2 * 36 * 6 * m * sum(len(x) for x in nbr)
+        out.append(sparse + 2 * (6 * m) ** 3)
+    return np.array(out, dtype=np.float64)
+
+
+def matvec_flops(scene) -> np.ndarray:
+    m = np.diff(scene.island_start).astype(np.float64)
+    return 2 * (6 * m) ** 2
+
+
+def cpu_baseline(target_s: float = 10.0):
+    """The oracle (C++ f64 restatement of the reference's algorithm) on a bounded
+    sample of the same workload: 64 chains of 8, warm steps until ~target_s."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle
+    from exgame_amd import scenes
+    threads = pyoracle.max_threads()
+    scene = scenes.jointed_chains(n_chains=64, m=M, seed=0)
+    w = pyoracle.OracleWorld(scene, tol2=TOL * TOL)
+    w.step(1, H, 1.0)  # the cold first step (seeding) is not part of the sample
+    n, t0 = 0, time.perf_counter()
+    while True:
+        w.step(1, H, 1.0)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= target_s or n >= 200:
+            break
+    return {"value": scene.n_bodies * n / el, "unit": "body-steps/s", "cores": threads, "kind": "port",
+            "sample": f"oracle, 64 chains x 8 bodies (512 bodies), {n} warm implicit steps after 1 cold, "
+                      f"{el:.1f} s on {threads} OpenMP threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (worlds shard by island with no "
+              f"exchange, gravity by target rows with one all-gather; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from exgame_amd import scenes, _lib
+    from exgame_amd.world import DeviceWorld
+
+    scene = scenes.jointed_chains(n_chains=CHAINS, m=M, seed=0)
+    info = _lib.device_info()
+
+    # warm-up on a throwaway world (code objects, caches, clocks)
+    if args.warmup > 0:
+        wu = DeviceWorld(scene, tol2=TOL * TOL)
+        wu.step(args.warmup, H, 1.0)
+        wu.sync()
+        wu.close()
+
+    world = DeviceWorld(scene, tol2=TOL * TOL)
+    world.sync()
+    stats0 = world.stats().astype(np.int64)
+    t0 = time.perf_counter()
+    world.step(args.steps, H, 1.0)
+    world.sync()
+    el = time.perf_counter() - t0
+    timing = world.last_timing()
+    stats = world.stats().astype(np.int64) - stats0
+
+    body_steps = scene.n_bodies * args.steps
+    value = body_steps / el
+    # roofline of the dominant kernel (the island solve): Newton–Schulz + matvec flops
+    ns_it = stats[:, 2].astype(np.float64)
+    newton_it = stats[:, 1].astype(np.float64)
+    flops = float(np.sum(ns_it * ns_flops_per_iteration(scene)) + np.sum(newton_it * matvec_flops(scene)))
+    island_s = timing["island_ms"] * 1e-3
+    per_launch_s = island_s / max(args.steps, 1)
+    achieved = flops / max(args.steps, 1) / per_launch_s / 1e12 if per_launch_s > 0 else 0.0
+
+    line = {
+        "metric": "body-steps/s",
+        "value": value,
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SplitMix64 scene)",
+        "config": {
+            "workload": "jointed_chains_implicit_f64: 4096 bodies = 512 chains x 8, axial k=1e3 c=10 + torsional "
+                        "c=5, N-body gravity G=1, h=1e-2, Newton tol 1e-10, NS 2/24, 200-step rollout",
+            "bodies": scene.n_bodies, "islands": scene.n_islands, "joints": scene.n_joints, "h": H,
+            "newton_tol": TOL, "device": info["name"], "cus": info["compute_units"],
+        },
+        "kernels_ms_per_step": {"gravity": timing["gravity_ms"] / args.steps,
+                                "island_solve": timing["island_ms"] / args.steps},
+        "solver_per_step": {"newton_iterations_per_island": float(newton_it.mean() / args.steps),
+                            "ns_iterations_per_island": float(ns_it.mean() / args.steps),
+                            "spans_per_island": float(stats[:, 0].mean() / args.steps)},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "island_step_kernel (Newton-Schulz + matvec flops / avg launch)"},
+    }
+    if not args.no_cpu_baseline:
+        try:
+            line["cpu_baseline"] = cpu_baseline()
+        except Exception as e:  # the baseline is reported beside the result, never instead of it
+            line["cpu_baseline"] = {"value": None, "error": str(e)}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -46,6 +46,8 @@ constexpr int MAX_ISLAND_JOINTS = 16;
 
 }  // namespace
 
+int exg::set_error(int code, const char* msg) { return fail(code, msg); }
+
 struct exg_world {
     int B = 0, I = 0, J = 0;
     double G = 1.0;
@@ -223,7 +225,8 @@ int exg_world_destroy(exg_world* w) {
 
 int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm, int ns_seed) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_solver: null world");
-    if (integrator != EXG_NEWTON) return fail(EXG_ERR_UNSUPPORTED, "exg_world_set_solver: only EXG_NEWTON runs on the device");
+    if (integrator < EXG_EXPLICIT_EULER || integrator > EXG_NEWTON)
+        return fail(EXG_ERR_INVALID, "exg_world_set_solver: unknown integrator");
     if (ns_warm < 0 || ns_seed < 1) return fail(EXG_ERR_INVALID, "exg_world_set_solver: bad Newton–Schulz budgets");
     w->integrator = integrator;
     w->tol2 = tol2;
@@ -281,7 +284,8 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         a.ns_warm = w->ns_warm;
         a.ns_seed = w->ns_seed;
         a.n_islands = w->I;
-        exg::launch_island_step(a, w->max_m, w->max_j, w->stream);
+        if (w->integrator == EXG_NEWTON) exg::launch_island_step(a, w->max_m, w->max_j, w->stream);
+        else exg::launch_explicit_step(a, w->integrator, w->stream);
         HIP_TRY(hipGetLastError());
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
         w->front = 1 - w->front;

@@ -4,6 +4,9 @@
 
 namespace exg {
 
+// Record the calling thread's last error (read back by exg_last_error) and return `code`.
+int set_error(int code, const char* msg);
+
 constexpr int ISLAND_THREADS = 256;
 constexpr int ISLAND_STATS = 8;  // spans, newton iterations, NS iterations, reseeds, line-search rejects, unconverged commits
 
@@ -32,6 +35,9 @@ struct IslandArgs {
 };
 
 void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream);
+
+// Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
+void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);
 
 // Pairwise gravity: ext[i] = (Σ_j F_ij, local_i × Σ_j F_ij), charges B×8.
 void launch_gravity_f64(const double* charges, int n, double g_inv, double* ext, hipStream_t stream);

@@ -123,6 +123,66 @@ int exg_world_stats(exg_world* w, int32_t* stats);
  * out[0] gravity, out[1] island solve (sums over the steps of that call). */
 int exg_world_last_timing(exg_world* w, double* out2);
 
+/* ───────────── typed slot storage and row-driven dispatch ─────────────
+ * The storage half of the boundary: one generational slot map per Pod type in HIP
+ * device memory, the analogue of aristotle::World with `usual` storages
+ * (crates/aristotle/src/world.rs:515-660). Keys are bit-exact with RawMap
+ * (world.rs:300-385): insert pops the lowest free slot or appends, growing the
+ * buffer ×2 (indices kept, storage generation +1); remove checks the generation,
+ * splices the slot into the address-ordered free list and bumps its generation. */
+enum exg_store_type {
+    EXG_T_SCALAR = 0,        /* T (f64)                         */
+    EXG_T_MOTOR = 1,         /* Motor: 8                        */
+    EXG_T_TWIST = 2,         /* Twist: 6                        */
+    EXG_T_WRENCH = 3,        /* Wrench: 6                       */
+    EXG_T_WRENCH_PAIR = 4,   /* [Wrench; 2]: 12                 */
+    EXG_T_JACOBIAN = 5,      /* [[Wrench; 24]; 2]: 288          */
+    EXG_T_BLOCK = 6,         /* Block = [[T; 6]; 6]: 36         */
+    EXG_T_DIAG_INERTIA = 7,  /* [T; 3]                          */
+    EXG_T_FULL_INERTIA = 8,  /* [[T; 3]; 3]                     */
+    EXG_T_ROW = 9            /* Row = [u32; 128] (rows/*.rs)    */
+};
+
+/* MessageKind numbering (crates/newton/src/accelerator/mod.rs:55-98). */
+enum exg_kind {
+    EXG_K_SIMPLE_SUM = 0,
+    EXG_K_CRITICALLY_DAMPED_WARPED_PLAIN = 1, EXG_K_CRITICALLY_DAMPED_WARPED_JACOBIAN = 2,
+    EXG_K_PERPENDICULAR_DAMPER_WARPED_PLAIN = 3, EXG_K_PERPENDICULAR_DAMPER_WARPED_JACOBIAN = 4,
+    EXG_K_SIMPLE_SPRING_DAMPER_PLAIN = 5, EXG_K_SIMPLE_SPRING_DAMPER_JACOBIAN = 6,
+    EXG_K_TORSIONAL_DAMPER_WARPED_PLAIN = 7, EXG_K_TORSIONAL_DAMPER_WARPED_JACOBIAN = 8,
+    EXG_K_PRE = 9, EXG_K_GATHER = 10, EXG_K_GEMM = 11,
+    EXG_K_BODY_POST_DIAGONAL = 12, EXG_K_BODY_POST_FULL = 13,
+    EXG_K_ASSEMBLE_BLOCK = 14, EXG_K_BLOCK_MATVEC = 15, EXG_K_BLOCK_REDUCE = 16, EXG_K_BLOCK_COPY = 17,
+    EXG_K_BODY_POST_GATHERED_DIAGONAL = 18, EXG_K_BODY_POST_GATHERED_FULL = 19
+};
+
+typedef struct exg_store exg_store;
+
+/* World::builder().capacity(c).usual() — every storage starts with `capacity` slots. */
+int exg_store_create(uint64_t capacity, exg_store** out);
+int exg_store_destroy(exg_store* s);
+/* WriteView::add → RawMap::insert: returns the slot index and its generation. */
+int exg_slot_insert(exg_store* s, int type, const void* value, uint32_t* index, uint32_t* generation);
+/* WorldKey::read / write / drop. `found` is 0 (and nothing is touched) for a stale,
+ * free or foreign key — RawMap::get/set/remove returning None. */
+int exg_slot_read(exg_store* s, int type, uint32_t index, uint32_t generation, void* out, int* found);
+int exg_slot_write(exg_store* s, int type, uint32_t index, uint32_t generation, const void* value, int* found);
+int exg_slot_remove(exg_store* s, int type, uint32_t index, uint32_t generation, void* value_out, int* found);
+/* Capacity, high-water length and storage generation (World::generation, world.rs:627-634). */
+int exg_storage_info(exg_store* s, int type, uint64_t* capacity, uint64_t* len, uint64_t* generation);
+int exg_storage_device_ptr(exg_store* s, int type, void** ptr);
+/* One round of rows of one MessageKind (Accelerator::submit_rounds → Shaders::dispatch,
+ * accelerator/mod.rs:900-964, shaders.rs:369-480). `row_slots` are indices into the
+ * ROW storage. Every slot a row names is validated first; two rows writing one slot
+ * are refused with EXG_ERR_BACKEND (the Ledger, shaders.rs:287-390). Completes before
+ * returning. */
+int exg_dispatch(exg_store* s, int kind, const uint32_t* row_slots, uint32_t count);
+/* Wait for queued work (the fence wait of a flush). */
+int exg_flush(exg_store* s);
+/* Fatal-slot count of the last dispatch of `kind` (always 0: the f64 kernels have
+ * no guarded-division sites). */
+int exg_fatals(exg_store* s, int kind, uint32_t* count);
+
 /* Stand-alone pairwise gravity on the device (gravity.rs:110-146, 530-601):
  * charges[n*8] in, wrench[n*6] out (force, local × force), host pointers. */
 int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench);

@@ -77,6 +77,21 @@ def test_chains_trajectory_matches_oracle():
     assert rel(dev.charges(), orc.front()) < 1e-9
 
 
+@pytest.mark.parametrize("integrator", ["symplectic_euler", "explicit_euler"])
+def test_plain_gravity_explicit_matches_oracle(integrator):
+    scene = scenes.plain_gravity(n=256, seed=0)
+    dev = DeviceWorld(scene, integrator=integrator)
+    orc = pyoracle.OracleWorld(scene, integrator=integrator)
+    dev.step(1, 1e-3, 1.0)
+    orc.step(1, 1e-3, 1.0)
+    assert rel(dev.state()["position"], orc.state()["position"]) < 1e-13
+    dev.step(29, 1e-3, 1.0)
+    orc.step(29, 1e-3, 1.0)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-11
+    assert rel(a["momentum"][:, :3], b["momentum"][:, :3]) < 1e-10
+
+
 def test_singletons_and_gravity_first_steps():
     scene = scenes.mixed_world(64, seed=5)
     dev, orc = run_pair(scene, 3)
