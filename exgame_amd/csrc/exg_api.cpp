@@ -57,7 +57,8 @@ struct exg_world {
     int ns_warm = 2, ns_seed = 24;
     int max_m = 0, max_j = 0;
     hipStream_t stream = nullptr;
-    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2];
+    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
+    exg::GravityPlan grav{};
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<long long> xacc_off;
     int front = 0;
@@ -67,7 +68,9 @@ struct exg_world {
     double last_ms[2] = {0.0, 0.0};
 
     void release() {
-        for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1]}) b->release();
+        for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
+                        &grav_partial})
+            b->release();
         for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
         xacc_off.release();
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -166,6 +169,12 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(w->origin.alloc(I * 3));
     W_TRY(w->charges[0].alloc(B * 8));
     W_TRY(w->charges[1].alloc(B * 8));
+    {
+        int cus = 0;
+        W_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+        w->grav = exg::gravity_plan(n_bodies, cus);
+        W_TRY(w->grav_partial.alloc(w->grav.partial_doubles));
+    }
     W_TRY(w->island_start.alloc(I + 1));
     W_TRY(w->island_jstart.alloc(I + 1));
     W_TRY(w->seeded.alloc(I));
@@ -252,7 +261,7 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         double* back = w->charges[1 - w->front].p;
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
         if (w->gravity) {
-            exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->ext.p, w->stream);
+            exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->ext.p, w->stream);
         } else {
             HIP_TRY(hipMemsetAsync(w->ext.p, 0, (size_t)w->B * 6 * sizeof(double), w->stream));
         }
@@ -348,21 +357,22 @@ int exg_world_stats(exg_world* w, int32_t* stats) {
 
 int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench) {
     if (n <= 0 || !charges || !wrench) return fail(EXG_ERR_INVALID, "exg_gravity_accumulate: bad arguments");
-    double *dc = nullptr, *dw = nullptr;
-    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&dc), (size_t)n * 8 * sizeof(double)));
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dw), (size_t)n * 6 * sizeof(double));
-    if (e != hipSuccess) {
-        (void)hipFree(dc);
-        return fail(EXG_ERR_BACKEND, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    e = hipMemcpy(dc, charges, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice);
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+    const exg::GravityPlan plan = exg::gravity_plan(n, cus);
+    double *dc = nullptr, *dw = nullptr, *dp = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dc), (size_t)n * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dw), (size_t)n * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dp), plan.partial_doubles * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpy(dc, charges, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        exg::launch_gravity_f64(dc, n, 1.0 / G, dw, nullptr);
+        exg::launch_gravity_f64(dc, n, 1.0 / G, plan, dp, dw, nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(wrench, dw, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
-    (void)hipFree(dc);
-    (void)hipFree(dw);
+    if (dc) (void)hipFree(dc);
+    if (dw) (void)hipFree(dw);
+    if (dp) (void)hipFree(dp);
     if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_gravity_accumulate: ") + hipGetErrorString(e));
     return EXG_OK;
 }

@@ -39,7 +39,15 @@ void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t s
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);
 
-// Pairwise gravity: ext[i] = (Σ_j F_ij, local_i × Σ_j F_ij), charges B×8.
-void launch_gravity_f64(const double* charges, int n, double g_inv, double* ext, hipStream_t stream);
+// Pairwise gravity: ext[i] = (Σ_j F_ij, local_i × Σ_j F_ij), charges B×8. The
+// sources are split into `chunks` chunks of `chunk_len`; partial holds
+// partial_doubles doubles of per-chunk sums.
+struct GravityPlan {
+    int tiles, chunks, chunk_len;
+    size_t partial_doubles;
+};
+GravityPlan gravity_plan(int n, int compute_units);
+void launch_gravity_f64(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
+                        double* ext, hipStream_t stream);
 
 }  // namespace exg

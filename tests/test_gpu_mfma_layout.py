@@ -85,8 +85,9 @@ def test_layout_is_a_permutation_and_multiplies():
 
 
 def test_documented_layout():
-    """lane l: A[l%16][l/16], B[l/16][l%16]; accumulator j of lane l holds the entry
-    the discovered map names, checked against the closed form the kernel uses."""
+    """lane l feeds A[l%16][l/16] and B[l/16][l%16]; accumulator j of lane l holds
+    C[l/16 + 4j][l%16] (measured on gfx950: the four accumulators of a lane are rows
+    4 apart, not 4 consecutive rows)."""
     a_of, b_of, c_of = discover()
     # relabel so A's row of lane l is l%16 and its depth l/16
     row_map = {a_of[l][0]: l % 16 for l in range(64)}
@@ -99,4 +100,4 @@ def test_documented_layout():
     print("C layout:", " ".join(f"{l}.{j}={r},{c}" for (l, j), (r, c) in sorted(table.items())))
     for l in range(64):
         for j in range(4):
-            assert table[(l, j)] == (4 * (l // 16) + j, l % 16), (l, j, table[(l, j)])
+            assert table[(l, j)] == (l // 16 + 4 * j, l % 16), (l, j, table[(l, j)])

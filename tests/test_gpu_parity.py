@@ -30,7 +30,7 @@ def random_charges(n, seed, coincident=True):
     return c
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 257, 600])
+@pytest.mark.parametrize("n", [1, 2, 3, 257, 600, 4096, 5003])
 def test_gravity_matches_oracle(n):
     c = random_charges(n, n, coincident=n > 8)
     got = gravity_accumulate(c, G=1.0)
@@ -41,6 +41,18 @@ def test_gravity_matches_oracle(n):
     else:
         assert rel(got[:, :3], want[:, :3]) < 1e-13
         assert rel(got[:, 3:], want[:, 3:]) < 1e-12
+
+
+def test_gravity_point_bodies_and_core_law():
+    # zero core sizes (pure 1/r²) mixed with cores larger than every distance
+    c = random_charges(300, 11)
+    c[::3, 7] = 0.0
+    c[1::7, 7] = 5.0
+    got = gravity_accumulate(c, G=0.7)
+    want = pyoracle.gravity_accumulate(c, G=0.7)
+    assert np.all(np.isfinite(got))
+    assert rel(got[:, :3], want[:, :3]) < 1e-13
+    assert rel(got[:, 3:], want[:, 3:]) < 1e-12
 
 
 def test_gravity_G_scaling_and_third_law():
