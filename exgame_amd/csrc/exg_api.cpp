@@ -55,6 +55,7 @@ struct exg_world {
     int integrator = EXG_NEWTON;
     double tol2 = 1e-8;
     int ns_warm = 2, ns_seed = 24;
+    int ns_engine = EXG_NS_MFMA;
     int max_m = 0, max_j = 0;
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
@@ -244,6 +245,14 @@ int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm,
     return EXG_OK;
 }
 
+int exg_world_set_ns_engine(exg_world* w, int engine) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_ns_engine: null world");
+    if (engine != EXG_NS_MFMA && engine != EXG_NS_VALU_ORDERED)
+        return fail(EXG_ERR_INVALID, "exg_world_set_ns_engine: unknown engine " + std::to_string(engine));
+    w->ns_engine = engine;
+    return EXG_OK;
+}
+
 int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_step: null world");
     if (n_steps < 0) return fail(EXG_ERR_INVALID, "exg_world_step: negative step count");
@@ -292,6 +301,7 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         a.G = w->G;
         a.ns_warm = w->ns_warm;
         a.ns_seed = w->ns_seed;
+        a.ns_engine = w->ns_engine;
         a.n_islands = w->I;
         if (w->integrator == EXG_NEWTON) exg::launch_island_step(a, w->max_m, w->max_j, w->stream);
         else exg::launch_explicit_step(a, w->integrator, w->stream);

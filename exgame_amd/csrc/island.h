@@ -25,13 +25,14 @@ struct IslandArgs {
     const int* ja;            // J, global body index
     const int* jb;            // J
     const double* jparams;    // J×10
-    double* xacc;             // accepted approximate inverses, m²·36 per island
+    double* xacc;             // accepted approximate inverses, (6m)² per island, row-major
     const long long* xacc_off;// I
     int* seeded;              // I
     double* charges_out;      // B×8: com xyz, local xyz, G·m, size
     int* stats;               // I×ISLAND_STATS
     double dt, warp, tol2, G;
     int ns_warm, ns_seed, n_islands;
+    int ns_engine;            // 0: Newton–Schulz products on MFMA; 1: VALU in the oracle's summation order
 };
 
 void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream);

@@ -24,6 +24,8 @@
 
 namespace exg {
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
 constexpr int NT = ISLAND_THREADS;
 constexpr int LANES = 8;
 constexpr int MAX_ITERATIONS = 32;
@@ -31,10 +33,17 @@ constexpr int MAX_DEPTH = 8;
 constexpr int THRESHOLD_LOG2 = 8;
 constexpr int PER_HALVING_LOG2 = 4;
 
+// The n×n system matrices (n = 6m) are stored flat, row-major, padded to n16 =
+// ⌈n/16⌉·16 rows and columns for the 16×16 MFMA tiles, with row stride LDM =
+// MAXN + 1 (odd in doubles): the sixteen rows an MFMA operand column spans then fall
+// on sixteen different LDS bank pairs. The padding stays zero.
 template <int MAXM, int MAXJ>
 struct IslandShared {
-    static constexpr int NB = MAXM * MAXM * 36;
+    static constexpr int MAXN = ((6 * MAXM + 15) / 16) * 16;
+    static constexpr int LDM = MAXN + 1;
+    static constexpr int NB = MAXN * LDM;
     double mat[4][NB];                      // A, X, X', R (X and X' swap by index)
+    unsigned kmask[MAXN / 16];              // per 16-row tile of A: 4-column slabs that can be non-zero
     double jac[MAXJ][2][24][6];             // ∂wrench[end]/∂axis, per joint
     double vmid[LANES][MAXM][6];
     double mid[LANES][MAXM][8];
@@ -76,7 +85,10 @@ template <int MAXM, int MAXJ>
 struct IslandSolver {
     IslandShared<MAXM, MAXJ>& s;
     const IslandArgs& a;
+    static constexpr int LD = IslandShared<MAXM, MAXJ>::LDM;
     int m, J, isl, first_body;
+    int n, n16;               // system size 6m and its 16-padded size
+    bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
     int ix, ixn;              // which of mat[1], mat[2] is X and which X'
     double dt, warp, half, span;
     long long xoff;
@@ -87,7 +99,6 @@ struct IslandSolver {
     __device__ double* X() { return s.mat[ix]; }
     __device__ double* Xn() { return s.mat[ixn]; }
     __device__ double* R() { return s.mat[3]; }
-    static constexpr int blk(int m_, int i, int j) { return (i * m_ + j) * 36; }
 
     // ───── PRE: midpoint = pose ∘ exp(half·vmid) for lanes [l0, l1) ─────
     __device__ void pre(int l0, int l1) {
@@ -176,9 +187,10 @@ struct IslandSolver {
     __device__ void assemble() {
         double pf = -(half * half), vf = -half;
         double* Am = A();
-        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
-            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
-            double acc = 0.0 * Am[e] + ((i == j) ? 1.0 : 0.0) * s.mass0[i][rc];
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            int row = e / n, col = e % n;
+            int i = row / 6, r = row % 6, j = col / 6, c = col % 6, b = i * m + j, rc = r * 6 + c;
+            double acc = 0.0 * Am[row * LD + col] + ((i == j) ? 1.0 : 0.0) * s.mass0[i][rc];
             for (int q = s.term_start[b]; q < s.term_start[b + 1]; ++q) {
                 int code = s.terms[q];
                 int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
@@ -186,49 +198,94 @@ struct IslandSolver {
                 double v = s.jac[jn][re][12 + ce * 6 + c][r];
                 acc = acc + pf * p + vf * v;
             }
-            Am[e] = acc;
+            Am[row * LD + col] = acc;
         }
     }
-    // ───── out = 0·out + Σ_k a[i,k]·(sign·b[k,j] + diag·2I) ─────
+    // ───── VALU form, the oracle's block order: out = 0·out + Σ_k a[i,k]·(sign·b[k,j] + diag·2I) ─────
     __device__ void gemm(const double* am, const double* bm, double* out, bool two_minus, bool sparse) {
         double sign = two_minus ? -1.0 : 1.0;
-        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
-            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
-            double acc = 0.0 * out[e];
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            int row = e / n, col = e % n, i = row / 6, j = col / 6, c = col % 6;
+            double acc = 0.0 * out[row * LD + col];
             int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : m;
             for (int q = k0; q < k1; ++q) {
                 int k = sparse ? s.nbr[q] : q;
                 double diag = (two_minus && k == j) ? 1.0 : 0.0;
-                const double* ab = am + blk(m, i, k) + r * 6;
-                const double* bb = bm + blk(m, k, j);
+                const double* ab = am + row * LD + 6 * k;
+                const double* bb = bm + 6 * k * LD + col;
                 double sum = 0.0;
                 for (int kk = 0; kk < 6; ++kk) {
-                    double right = sign * bb[kk * 6 + c];
+                    double right = sign * bb[kk * LD];
                     if (kk == c) right = right + diag * 2.0;
                     sum = sum + ab[kk] * right;
                 }
                 acc = acc + sum;
             }
-            out[e] = acc;
+            out[row * LD + col] = acc;
         }
     }
-    // ───── Σ ‖diag·I − b‖² over every block of `mat` (n_blocks blocks) ─────
-    __device__ double reduce_sq(const double* mat, int n_blocks, bool identity_diag) {
+    // ───── MFMA form: out = a·b over 16×16 tiles, k in 4-column slabs ─────
+    // Wave w computes tiles w, w+4, … of the n16×n16 product; a slab whose bit is
+    // clear in kmask[tile row] is all-zero in `a` and skipped. With make_t the
+    // epilogue stores T = 2I − a·b (what the second Newton–Schulz product needs) and
+    // returns this thread's share of ‖I − a·b‖². Padding rows/columns are stored 0.
+    // Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l supplies
+    // a[l%16][l/16] and b[l/16][l%16]; accumulator j holds c[l/16 + 4j][l%16].
+    __device__ double mfma_product(const double* am, const double* bm, double* out, const unsigned* kmask,
+                                   bool make_t) {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int lr = lane & 15, lk = lane >> 4;
+        const int T = n16 >> 4, KS = n16 >> 2;
+        const unsigned dense = (1u << KS) - 1u;
         double part = 0.0;
-        for (int e = threadIdx.x; e < n_blocks * 36; e += NT) {
-            int b = e / 36, rc = e % 36, r = rc / 6, c = rc % 6;
-            double diag = (identity_diag && (b / m == b % m)) ? 1.0 : 0.0;
-            double v = mat[e];
-            double x = (r == c) ? diag - v : -v;
+        for (int t = wave; t < T * T; t += NT / 64) {
+            const int ti = t / T, tj = t % T;
+            const unsigned mk = kmask ? kmask[ti] : dense;
+            const double* ap = am + (ti * 16 + lr) * LD + lk;
+            const double* bp = bm + lk * LD + tj * 16 + lr;
+            d4 acc = {0.0, 0.0, 0.0, 0.0};
+            for (int ks = 0; ks < KS; ++ks)
+                if (mk & (1u << ks)) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[4 * ks], bp[4 * ks * LD], acc, 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                const int r = ti * 16 + lk + 4 * j, c = tj * 16 + lr;
+                const bool real = r < n && c < n;
+                double v = acc[j];
+                if (make_t) {
+                    const double d = (r == c) ? 1.0 : 0.0;
+                    const double x = d - v;
+                    part += real ? x * x : 0.0;
+                    v = 2.0 * d - v;
+                }
+                out[r * LD + c] = real ? v : 0.0;
+            }
+        }
+        return part;
+    }
+    // ───── Σ ‖diag·I − b‖² over the n×n entries of `mat` ─────
+    __device__ double reduce_sq(const double* mat, bool identity_diag) {
+        double part = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            int row = e / n, col = e % n;
+            double diag = (identity_diag && row == col) ? 1.0 : 0.0;
+            double v = mat[row * LD + col];
+            double x = diag - v;
             part = part + x * x;
         }
         return block_sum(s, part);
     }
     __device__ double refine_inverse() {  // newton.rs:255-279
-        gemm(A(), X(), R(), false, true);
-        __syncthreads();
-        double before = reduce_sq(R(), m * m, true);
-        gemm(X(), R(), Xn(), true, false);
+        double before;
+        if (mfma) {
+            // R ← T = 2I − A·X (A block-sparse: zero slabs skipped), then X' = X·T
+            double part = mfma_product(A(), X(), R(), s.kmask, true);
+            before = block_sum(s, part);  // its barrier also publishes T
+            mfma_product(X(), R(), Xn(), nullptr, false);
+        } else {
+            gemm(A(), X(), R(), false, true);
+            __syncthreads();
+            before = reduce_sq(R(), true);
+            gemm(X(), R(), Xn(), true, false);
+        }
         __syncthreads();
         int t = ix;
         ix = ixn;
@@ -242,10 +299,11 @@ struct IslandSolver {
             int i = threadIdx.x % m;
             bool col = threadIdx.x >= m;
             double sum = 0.0;
-            for (int j = 0; j < m; ++j) {
-                const double* b = A() + (col ? blk(m, j, i) : blk(m, i, j));
-                for (int k = 0; k < 36; ++k) sum += fabs(b[k]);
-            }
+            const double* Am = A();
+            for (int j = 0; j < m; ++j)
+                for (int r = 0; r < 6; ++r)
+                    for (int c = 0; c < 6; ++c)
+                        sum += fabs(col ? Am[(6 * j + r) * LD + 6 * i + c] : Am[(6 * i + r) * LD + 6 * j + c]);
             s.red[threadIdx.x] = sum;
         }
         __syncthreads();
@@ -260,9 +318,9 @@ struct IslandSolver {
         __syncthreads();
         double sc = s.bcast[0];
         double* Xm = X();
-        for (int e = threadIdx.x; e < m * m * 36; e += NT) {
-            int b = e / 36, rc = e % 36, i = b / m, j = b % m, r = rc / 6, c = rc % 6;
-            Xm[e] = A()[blk(m, j, i) + c * 6 + r] * sc;
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            int row = e / n, col = e % n;
+            Xm[row * LD + col] = A()[col * LD + row] * sc;
         }
         __syncthreads();
         seeded = true;
@@ -289,11 +347,11 @@ struct IslandSolver {
     // ───── dv = 0·dv + Σ_j X[i,j]·rhs_j ─────
     __device__ void matvec() {
         const double* Xm = X();
-        for (int t = threadIdx.x; t < m * 6; t += NT) {
+        for (int t = threadIdx.x; t < n; t += NT) {
             int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
             for (int j = 0; j < m; ++j) {
-                const double* xb = Xm + blk(m, i, j) + r * 6;
+                const double* xb = Xm + t * LD + 6 * j;
                 double sum = 0.0;
                 for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.rhs[0][j][c];
                 acc = acc + sum;
@@ -437,7 +495,7 @@ struct IslandSolver {
     }
     __device__ bool signal_log2(bool assembled, int& r) {
         if (!assembled) return false;
-        double an = reduce_sq(A(), m * m, false);
+        double an = reduce_sq(A(), false);
         double mn = 0.0;
         {
             double part = 0.0;
@@ -453,16 +511,16 @@ struct IslandSolver {
     __device__ void restore_x() {
         double* Xm = X();
         const double* src = a.xacc + xoff;
-        for (int e = threadIdx.x; e < m * m * 36; e += NT) Xm[e] = src[e];
+        for (int e = threadIdx.x; e < n * n; e += NT) Xm[(e / n) * LD + e % n] = src[e];
         __syncthreads();
     }
     __device__ void publish_x(bool& accepted_seeded) {
         const double* Xm = X();
         double* dst = a.xacc + xoff;
-        for (int e = threadIdx.x; e < m * m * 36; e += NT) dst[e] = Xm[e];
-        double n = reduce_sq(Xm, m * m, false);
+        for (int e = threadIdx.x; e < n * n; e += NT) dst[e] = Xm[(e / n) * LD + e % n];
+        double nx = reduce_sq(Xm, false);
         const double huge = 1000000000.0;
-        if (n < huge * huge) accepted_seeded = seeded;
+        if (nx < huge * huge) accepted_seeded = seeded;
         else { seeded = false; accepted_seeded = false; }
     }
     static __device__ int floor_div(int q, int d) { return (q >= 0) ? q / d : -((-q + d - 1) / d); }
@@ -548,12 +606,16 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
         for (int p = 0; p < 10; ++p) s.jparams[t][p] = a.jparams[(size_t)(j0 + t) * 10 + p];
     }
     const long long xoff = a.xacc_off[isl];
-    for (int e = threadIdx.x; e < m * m * 36; e += NT) {
+    constexpr int LD = IslandShared<MAXM, MAXJ>::LDM;
+    const int n = 6 * m;
+    for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ>::NB; e += NT) {
         s.mat[0][e] = 0.0;
-        s.mat[1][e] = a.xacc[xoff + e];
+        s.mat[1][e] = 0.0;
         s.mat[2][e] = 0.0;
         s.mat[3][e] = 0.0;
     }
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * n; e += NT) s.mat[1][(e / n) * LD + e % n] = a.xacc[xoff + e];
     __syncthreads();
     // ── topology tables: neighbours, assembly terms, incidence (joint order) ──
     if (threadIdx.x == 0) {
@@ -598,12 +660,23 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
             }
         }
         s.inc_start[m] = o;
+        // A's non-zero 4-column slabs per 16-row tile, from the block neighbour lists
+        for (int t = 0; t < IslandShared<MAXM, MAXJ>::MAXN / 16; ++t) s.kmask[t] = 0u;
+        for (int bi = 0; bi < m; ++bi)
+            for (int q = s.nbr_start[bi]; q < s.nbr_start[bi + 1]; ++q) {
+                int bk = s.nbr[q];
+                for (int t = (6 * bi) / 16; t <= (6 * bi + 5) / 16; ++t)
+                    for (int ks = (6 * bk) / 4; ks <= (6 * bk + 5) / 4; ++ks) s.kmask[t] |= 1u << ks;
+            }
     }
     __syncthreads();
 
     IslandSolver<MAXM, MAXJ> sv{s, a};
     sv.m = m;
     sv.J = J;
+    sv.n = n;
+    sv.n16 = ((n + 15) / 16) * 16;
+    sv.mfma = a.ns_engine == 0;
     sv.isl = isl;
     sv.first_body = b0;
     sv.ix = 1;

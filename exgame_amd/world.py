@@ -15,6 +15,8 @@ import numpy as np
 from . import _lib, pga
 
 INTEGRATORS = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2}
+# Newton–Schulz products: FP64 MFMA tiles, or scalar FMAs in the reference's block order
+NS_ENGINES = {"mfma": 0, "valu_ordered": 1}
 
 
 @dataclass(frozen=True)
@@ -26,7 +28,7 @@ class Epoch:
 
 class DeviceWorld:
     def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
-                 ns_warm: int = 2, ns_seed: int = 24):
+                 ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma"):
         L = _lib.lib()
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
@@ -57,6 +59,7 @@ class DeviceWorld:
             int(bool(gravity)), C.byref(h)))
         self.h = h
         _lib.check(L.exg_world_set_solver(self.h, INTEGRATORS[integrator], float(tol2), int(ns_warm), int(ns_seed)))
+        _lib.check(L.exg_world_set_ns_engine(self.h, NS_ENGINES[ns_engine]))
 
     def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
         if isinstance(dt, Epoch):

@@ -97,6 +97,13 @@ int exg_world_destroy(exg_world* w);
  * (newton.rs:139-151, 802-805), Newton–Schulz warm/cold budgets (newton.rs:58-61). */
 int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm, int ns_seed);
 
+/* How the Newton–Schulz products run (no counterpart in the reference, whose GEMM rows
+ * are one GLSL kernel, build.rs:1156-1403): EXG_NS_MFMA (default) uses FP64 MFMA
+ * tiles; EXG_NS_VALU_ORDERED uses scalar FMAs in the reference's block summation order,
+ * so results match the reference's arithmetic term for term. Both are f64. */
+enum { EXG_NS_MFMA = 0, EXG_NS_VALU_ORDERED = 1 };
+int exg_world_set_ns_engine(exg_world* w, int engine);
+
 /* Advance `n_steps` steps of length dt with time warp `warp` (aristotle Epoch dt/warp,
  * crates/aristotle/src/epoch.rs:238-248): per step, gravity from the charges published
  * by the previous step (zero on the first, gravity.rs:620-626), the island solve, the

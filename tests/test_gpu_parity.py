@@ -62,17 +62,23 @@ def test_gravity_G_scaling_and_third_law():
     assert np.abs(w.sum(0)).max() < 1e-12 * np.abs(w).max()
 
 
-def run_pair(scene, steps, dt=1e-2, tol2=1e-20):
-    dev = DeviceWorld(scene, tol2=tol2)
+def run_pair(scene, steps, dt=1e-2, tol2=1e-20, ns_engine="mfma"):
+    dev = DeviceWorld(scene, tol2=tol2, ns_engine=ns_engine)
     orc = pyoracle.OracleWorld(scene, tol2=tol2)
     dev.step(steps, dt, 1.0)
     orc.step(steps, dt, 1.0)
     return dev, orc
 
 
-def test_chain_first_step_matches_oracle():
+ENGINES = ["mfma", "valu_ordered"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_chain_first_step_matches_oracle(engine):
+    # the VALU engine sums in the oracle's order; the MFMA engine differs by rounding
+    # only, so the solver's decisions (iteration counts, reseeds) are the same
     scene = scenes.jointed_chains(n_chains=4, m=8, seed=0)
-    dev, orc = run_pair(scene, 1)
+    dev, orc = run_pair(scene, 1, ns_engine=engine)
     a, b = dev.state(), orc.state()
     assert rel(a["position"], b["position"]) < 1e-12
     assert rel(a["momentum"], b["momentum"]) < 1e-12
@@ -80,9 +86,29 @@ def test_chain_first_step_matches_oracle():
     np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
 
 
-def test_chains_trajectory_matches_oracle():
+@pytest.mark.parametrize("m", [2, 3, 5, 7])
+def test_chain_lengths_first_steps(m):
+    # n = 6m from 12 to 42: one to three 16-row MFMA tiles with zero padding
+    scene = scenes.jointed_chains(n_chains=6, m=m, seed=m)
+    dev, orc = run_pair(scene, 2)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-12
+    np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
+
+
+def test_rigs_and_chains_first_steps():
+    scene = scenes.mixed_world(160, seed=9, rigs=True)
+    dev, orc = run_pair(scene, 2)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-11
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_chains_trajectory_matches_oracle(engine):
     scene = scenes.jointed_chains(n_chains=16, m=8, seed=3)
-    dev, orc = run_pair(scene, 20)
+    dev, orc = run_pair(scene, 20, ns_engine=engine)
     a, b = dev.state(), orc.state()
     assert rel(a["position"], b["position"]) < 1e-9
     assert rel(a["momentum"], b["momentum"]) < 1e-9
