@@ -81,6 +81,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="before the run, copy this many MiB device-to-device once (the known-size copy that "
+                         "scales the HBM byte counters of a profile)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (worlds shard by island with no "
@@ -92,6 +95,14 @@ def main() -> int:
 
     scene = scenes.jointed_chains(n_chains=CHAINS, m=M, seed=0)
     info = _lib.device_info()
+    if args.copy_calibration_mib > 0:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # warm-up on a throwaway world (code objects, caches, clocks)
     if args.warmup > 0:

@@ -51,6 +51,8 @@ SIGNATURES = {
     "exg_world_stats": [C.c_void_p, _i32p],
     "exg_world_last_timing": [C.c_void_p, _dp],
     "exg_gravity_accumulate": [C.c_int, _dp, C.c_double, _dp],
+    "exg_gravity_accumulate_precision": [C.c_int, _dp, C.c_double, C.c_int, _dp],
+    "exg_world_set_precision": [C.c_void_p, C.c_int, C.c_int],
 }
 
 _lib = None

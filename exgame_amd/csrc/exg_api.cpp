@@ -59,7 +59,9 @@ struct exg_world {
     int max_m = 0, max_j = 0;
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
+    DevBuf<float> grav_narrow;
     exg::GravityPlan grav{};
+    int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<long long> xacc_off;
     int front = 0;
@@ -73,6 +75,7 @@ struct exg_world {
                         &grav_partial})
             b->release();
         for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
+        grav_narrow.release();
         xacc_off.release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -245,6 +248,21 @@ int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm,
     return EXG_OK;
 }
 
+int exg_world_set_precision(exg_world* w, int gravity, int jacobian) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_precision: null world");
+    auto known = [](int p) { return p == EXG_PREC_F64 || p == EXG_PREC_F32_COMPENSATED; };
+    if (!known(gravity) || !known(jacobian)) return fail(EXG_ERR_INVALID, "exg_world_set_precision: unknown precision");
+    if (jacobian != EXG_PREC_F64)
+        return fail(EXG_ERR_UNSUPPORTED, "exg_world_set_precision: f32 Jacobian assembly is not available");
+    if (gravity == EXG_PREC_F32_COMPENSATED && !w->grav_narrow.p) {
+        hipError_t e = w->grav_narrow.alloc((size_t)w->B * 6);
+        if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_world_set_precision: ") + hipGetErrorString(e));
+    }
+    w->grav_prec = gravity;
+    w->jac_prec = jacobian;
+    return EXG_OK;
+}
+
 int exg_world_set_ns_engine(exg_world* w, int engine) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_ns_engine: null world");
     if (engine != EXG_NS_MFMA && engine != EXG_NS_VALU_ORDERED)
@@ -270,7 +288,11 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         double* back = w->charges[1 - w->front].p;
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
         if (w->gravity) {
-            exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->ext.p, w->stream);
+            if (w->grav_prec == EXG_PREC_F32_COMPENSATED)
+                exg::launch_gravity_f32(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->grav_narrow.p, w->ext.p,
+                                        w->stream);
+            else
+                exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->ext.p, w->stream);
         } else {
             HIP_TRY(hipMemsetAsync(w->ext.p, 0, (size_t)w->B * 6 * sizeof(double), w->stream));
         }
@@ -366,23 +388,34 @@ int exg_world_stats(exg_world* w, int32_t* stats) {
 }
 
 int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench) {
+    return exg_gravity_accumulate_precision(n, charges, G, EXG_PREC_F64, wrench);
+}
+
+int exg_gravity_accumulate_precision(int n, const double* charges, double G, int precision, double* wrench) {
     if (n <= 0 || !charges || !wrench) return fail(EXG_ERR_INVALID, "exg_gravity_accumulate: bad arguments");
+    if (precision != EXG_PREC_F64 && precision != EXG_PREC_F32_COMPENSATED)
+        return fail(EXG_ERR_INVALID, "exg_gravity_accumulate: unknown precision");
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
     const exg::GravityPlan plan = exg::gravity_plan(n, cus);
     double *dc = nullptr, *dw = nullptr, *dp = nullptr;
+    float* dn = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&dc), (size_t)n * 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dw), (size_t)n * 6 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dp), plan.partial_doubles * sizeof(double));
+    if (e == hipSuccess && precision == EXG_PREC_F32_COMPENSATED)
+        e = hipMalloc(reinterpret_cast<void**>(&dn), (size_t)n * 6 * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(dc, charges, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        exg::launch_gravity_f64(dc, n, 1.0 / G, plan, dp, dw, nullptr);
+        if (precision == EXG_PREC_F32_COMPENSATED) exg::launch_gravity_f32(dc, n, 1.0 / G, plan, dp, dn, dw, nullptr);
+        else exg::launch_gravity_f64(dc, n, 1.0 / G, plan, dp, dw, nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(wrench, dw, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
     if (dc) (void)hipFree(dc);
     if (dw) (void)hipFree(dw);
     if (dp) (void)hipFree(dp);
+    if (dn) (void)hipFree(dn);
     if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_gravity_accumulate: ") + hipGetErrorString(e));
     return EXG_OK;
 }

@@ -114,6 +114,83 @@ __global__ __launch_bounds__(GRAV_T) void gravity_reduce_kernel(const double* __
     o[5] = lx * f[1] - ly * f[0];
 }
 
+// ───── mixed precision: FP32 pair arithmetic, two-sum compensated FP32 accumulators ─────
+// Each step the f64 charges are narrowed once to (x, y, z, G·m) and (s², 1/s) in
+// f32; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
+// component is accumulated as an unevaluated sum hi + lo: after every addition
+// the rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to
+// lo. The per-chunk sums are widened to f64 for the ordered chunk reduction, the
+// scaling by the target's charge and the moment, which stay f64.
+__device__ inline void two_sum_acc(float& hi, float& lo, float x) {
+    const float t = hi + x;
+    const float bp = t - hi;
+    const float e = (hi - (t - bp)) + (x - bp);
+    hi = t;
+    lo = lo + e;
+}
+
+__global__ __launch_bounds__(GRAV_T) void gravity_narrow_kernel(const double* __restrict__ ch, int n,
+                                                                float4* __restrict__ pos_g,
+                                                                float2* __restrict__ core) {
+    const int i = blockIdx.x * GRAV_T + threadIdx.x;
+    if (i >= n) return;
+    const double* c = ch + (size_t)i * 8;
+    pos_g[i] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
+    const float sz = (float)c[7];
+    core[i] = make_float2(sz * sz, 1.0f / sz);
+}
+
+__global__ __launch_bounds__(GRAV_T) void gravity_partial_f32_kernel(const float4* __restrict__ pos_g,
+                                                                     const float2* __restrict__ core, int n,
+                                                                     int chunk_len, double* __restrict__ partial) {
+    __shared__ float4 sp[GRAV_T];
+    __shared__ float2 sc[GRAV_T];
+    const int i = blockIdx.x * GRAV_T + threadIdx.x;
+    const int chunk = blockIdx.y;
+    const int c0 = chunk * chunk_len;
+    const int c1 = min(n, c0 + chunk_len);
+    const bool live = i < n;
+    float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 mc = make_float2(1.f, 1.f);
+    if (live) {
+        me = pos_g[i];
+        mc = core[i];
+    }
+    float fx = 0.f, fy = 0.f, fz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
+    for (int base = c0; base < c1; base += GRAV_T) {
+        const int j = base + threadIdx.x;
+        if (j < c1) {
+            sp[threadIdx.x] = pos_g[j];
+            sc[threadIdx.x] = core[j];
+        }
+        __syncthreads();
+        const int cnt = min(GRAV_T, c1 - base);
+        if (live) {
+            for (int q = 0; q < cnt; ++q) {
+                const int jj = base + q;
+                const float4 o = sp[q];
+                const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                const bool hi_is_j = jj > i;
+                const float2 cr = hi_is_j ? sc[q] : mc;
+                const float inv_r = (d2 >= cr.x) ? __builtin_amdgcn_rsqf(d2) : cr.y;
+                float w = o.w * (inv_r * inv_r * inv_r);
+                w = (jj == i || d2 < 1e-18f) ? 0.f : w;
+                two_sum_acc(fx, ex, w * dx);
+                two_sum_acc(fy, ey, w * dy);
+                two_sum_acc(fz, ez, w * dz);
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* p = partial + (size_t)chunk * 3 * n;
+        p[i] = (double)fx + (double)ex;
+        p[(size_t)n + i] = (double)fy + (double)ey;
+        p[(size_t)2 * n + i] = (double)fz + (double)ez;
+    }
+}
+
 GravityPlan gravity_plan(int n, int compute_units) {
     GravityPlan p{};
     if (n <= 0) return p;
@@ -128,6 +205,18 @@ GravityPlan gravity_plan(int n, int compute_units) {
     p.tiles = tiles;
     p.partial_doubles = (size_t)p.chunks * 3 * n;
     return p;
+}
+
+void launch_gravity_f32(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
+                        float* narrow, double* ext, hipStream_t stream) {
+    if (n <= 0) return;
+    float4* pos_g = reinterpret_cast<float4*>(narrow);
+    float2* core = reinterpret_cast<float2*>(narrow + (size_t)4 * n);
+    hipLaunchKernelGGL(gravity_narrow_kernel, dim3(plan.tiles), dim3(GRAV_T), 0, stream, charges, n, pos_g, core);
+    hipLaunchKernelGGL(gravity_partial_f32_kernel, dim3(plan.tiles, plan.chunks), dim3(GRAV_T), 0, stream, pos_g,
+                       core, n, plan.chunk_len, partial);
+    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(plan.tiles), dim3(GRAV_T), 0, stream, charges, n, plan.chunks,
+                       g_inv, partial, ext);
 }
 
 void launch_gravity_f64(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,

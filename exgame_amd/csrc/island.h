@@ -50,5 +50,9 @@ struct GravityPlan {
 GravityPlan gravity_plan(int n, int compute_units);
 void launch_gravity_f64(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
                         double* ext, hipStream_t stream);
+// Mixed precision: FP32 pairs with compensated accumulation; `narrow` holds 6n floats
+// (x, y, z, G·m | s², 1/s per body, 16-byte aligned).
+void launch_gravity_f32(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
+                        float* narrow, double* ext, hipStream_t stream);
 
 }  // namespace exg

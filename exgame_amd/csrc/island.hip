@@ -26,6 +26,18 @@ namespace exg {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
+// Wave-uniform copies of values every lane already holds (LDS broadcasts, wave
+// indices): they live in scalar registers, so branches on them are scalar branches
+// instead of exec-masked vector ones.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline unsigned uni(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ inline double uni(double v) {
+    long long b = __double_as_longlong(v);
+    unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 constexpr int NT = ISLAND_THREADS;
 constexpr int LANES = 8;
 constexpr int MAX_ITERATIONS = 32;
@@ -52,6 +64,7 @@ struct IslandShared {
     double scale[LANES][MAXM][2];
     double conn[LANES][MAXJ][2][6];
     double wn[LANES];
+    double ratio[LANES][MAXM][2];
     double mass0[MAXM][36];                 // lane 0's world mass block
     double pose[MAXM][8], mom[MAXM][6], massv[MAXM], tens[MAXM][9], ext[MAXM][6];
     double snap[MAXM][6], dv[MAXM][6], base[MAXM][6], best[MAXM][6];
@@ -59,26 +72,28 @@ struct IslandShared {
     double red[NT];
     double bcast[8];
     int jkind[MAXJ], ja[MAXJ], jb[MAXJ];
+    int jorder[MAXJ];                       // joints grouped by kind (thread mapping only)
     int nbr_start[MAXM + 1], nbr[MAXM * MAXM];
     int term_start[MAXM * MAXM + 1], terms[4 * MAXJ];
     int inc_start[MAXM + 1], inc[2 * MAXJ];
     int ibcast[8];
 };
 
-// Block-wide sum of one value per thread (tree order; the consumers compare the
-// result against a ×4 margin or take ⌊log₂⌋, never test it for equality).
+// Block-wide sum of one value per thread: a butterfly within each 64-lane wave,
+// then the four wave sums added in wave order by every thread (the consumers compare
+// the result against a ×4 margin or take ⌊log₂⌋, never test it for equality). The
+// leading barrier protects s.red from the previous call's readers.
 template <int MAXM, int MAXJ>
 __device__ double block_sum(IslandShared<MAXM, MAXJ>& s, double v) {
-    int t = threadIdx.x;
-    s.red[t] = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-        if (t < w) s.red[t] = s.red[t] + s.red[t + w];
-        __syncthreads();
-    }
+    if ((threadIdx.x & 63) == 0) s.red[threadIdx.x >> 6] = v;
+    __syncthreads();
     double r = s.red[0];
-    __syncthreads();
-    return r;
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w) r = r + s.red[w];
+    return uni(r);
 }
 
 template <int MAXM, int MAXJ>
@@ -113,7 +128,7 @@ struct IslandSolver {
     __device__ void joints_jacobian() {
         using D = Dual<double>;
         for (int t = threadIdx.x; t < J * 24; t += NT) {
-            int j = t / 24, ax = t % 24;
+            int j = s.jorder[t / 24], ax = t % 24;
             int ends[2] = {s.ja[j], s.jb[j]};
             D pp[2][8], vv[2][6];
             for (int e = 0; e < 2; ++e) {
@@ -138,8 +153,9 @@ struct IslandSolver {
     }
     // ───── value-only joints for lanes [l0, l1) ─────
     __device__ void joints_plain(int l0, int l1) {
-        for (int t = threadIdx.x; t < (l1 - l0) * J; t += NT) {
-            int lane = l0 + t / J, j = t % J;
+        const int nl = l1 - l0;
+        for (int t = threadIdx.x; t < nl * J; t += NT) {
+            int lane = l0 + t % nl, j = s.jorder[t / nl];
             joint_wrench(s.jkind[j], s.jparams[j], s.mid[lane][s.ja[j]], s.mid[lane][s.jb[j]], s.vmid[lane][s.ja[j]],
                          s.vmid[lane][s.jb[j]], dt, warp, s.conn[lane][j][0], s.conn[lane][j][1]);
         }
@@ -170,15 +186,25 @@ struct IslandSolver {
     }
     static __device__ double max2(double x, double y) { return x > y ? x : y; }
     // ───── worst² residual ratio per lane (newton.rs:139-151) ─────
+    // Two levels, the fold order unchanged: each (lane, body) thread forms its
+    // body's two ratios; after a barrier one thread per lane folds the bodies in
+    // order with the same max2 (so NaN handling matches the sequential form).
     __device__ void wnorm(int l0, int l1) {
+        const int nl = l1 - l0;
+        for (int t = threadIdx.x; t < nl * m; t += NT) {
+            const int lane = l0 + t / m, i = t % m;
+            const double* r = s.rhs[lane][i];
+            double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
+            s.ratio[lane][i][0] = lin2 / s.scale[lane][i][0];
+            s.ratio[lane][i][1] = ang2 / s.scale[lane][i][1];
+        }
+        __syncthreads();
         for (int lane = l0 + (int)threadIdx.x; lane < l1; lane += NT) {
             double worst = 0.0;
             for (int i = 0; i < m; ++i) {
-                const double* r = s.rhs[lane][i];
-                double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-                double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
-                worst = max2(worst, lin2 / s.scale[lane][i][0]);
-                worst = max2(worst, ang2 / s.scale[lane][i][1]);
+                worst = max2(worst, s.ratio[lane][i][0]);
+                worst = max2(worst, s.ratio[lane][i][1]);
             }
             s.wn[lane] = worst;
         }
@@ -187,18 +213,26 @@ struct IslandSolver {
     __device__ void assemble() {
         double pf = -(half * half), vf = -half;
         double* Am = A();
-        for (int e = threadIdx.x; e < n * n; e += NT) {
-            int row = e / n, col = e % n;
-            int i = row / 6, r = row % 6, j = col / 6, c = col % 6, b = i * m + j, rc = r * 6 + c;
-            double acc = 0.0 * Am[row * LD + col] + ((i == j) ? 1.0 : 0.0) * s.mass0[i][rc];
-            for (int q = s.term_start[b]; q < s.term_start[b + 1]; ++q) {
-                int code = s.terms[q];
-                int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
-                double p = s.jac[jn][re][ce * 6 + c][r];
-                double v = s.jac[jn][re][12 + ce * 6 + c][r];
-                acc = acc + pf * p + vf * v;
+        for (int e = threadIdx.x; e < m * m * 6; e += NT) {
+            const int b = e / 6, r = e - 6 * b, i = b / m, j = b - i * m;
+            const int q0 = s.term_start[b], q1 = s.term_start[b + 1];
+            const double dg = (i == j) ? 1.0 : 0.0;
+            double* dst = Am + (6 * i + r) * LD + 6 * j;
+            double acc[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) acc[c] = 0.0 * dst[c] + dg * s.mass0[i][r * 6 + c];
+            for (int q = q0; q < q1; ++q) {
+                const int code = s.terms[q];
+                const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    double p = s.jac[jn][re][ce * 6 + c][r];
+                    double v = s.jac[jn][re][12 + ce * 6 + c][r];
+                    acc[c] = acc[c] + pf * p + vf * v;
+                }
             }
-            Am[row * LD + col] = acc;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) dst[c] = acc[c];
         }
     }
     // ───── VALU form, the oracle's block order: out = 0·out + Σ_k a[i,k]·(sign·b[k,j] + diag·2I) ─────
@@ -233,23 +267,62 @@ struct IslandSolver {
     // a[l%16][l/16] and b[l/16][l%16]; accumulator j holds c[l/16 + 4j][l%16].
     __device__ double mfma_product(const double* am, const double* bm, double* out, const unsigned* kmask,
                                    bool make_t) {
-        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
         const unsigned dense = (1u << KS) - 1u;
+        constexpr int MAXKS = IslandShared<MAXM, MAXJ>::MAXN / 4;
+        constexpr int MAXT = IslandShared<MAXM, MAXJ>::MAXN / 16;
+        constexpr int PER_WAVE = (MAXT * MAXT + NT / 64 - 1) / (NT / 64);
         double part = 0.0;
-        for (int t = wave; t < T * T; t += NT / 64) {
-            const int ti = t / T, tj = t % T;
-            const unsigned mk = kmask ? kmask[ti] : dense;
-            const double* ap = am + (ti * 16 + lr) * LD + lk;
-            const double* bp = bm + lk * LD + tj * 16 + lr;
-            d4 acc = {0.0, 0.0, 0.0, 0.0};
-            for (int ks = 0; ks < KS; ++ks)
-                if (mk & (1u << ks)) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(ap[4 * ks], bp[4 * ks * LD], acc, 0, 0, 0);
+        // this wave's tiles: wave, wave + 4, … (wave-uniform), accumulated side by side
+        int ti[PER_WAVE], tj[PER_WAVE];
+        unsigned mk[PER_WAVE];
+        d4 acc[PER_WAVE];
+#pragma unroll
+        for (int u = 0; u < PER_WAVE; ++u) {
+            const int t = wave + u * (NT / 64);
+            const bool live = t < T * T;
+            ti[u] = live ? t / T : 0;
+            tj[u] = live ? t % T : 0;
+            mk[u] = live ? (kmask ? uni(kmask[ti[u]]) : dense) : 0u;
+            acc[u] = d4{0.0, 0.0, 0.0, 0.0};
+        }
+        // operands of slab ks+1 are loaded while slab ks multiplies (all loads stay
+        // inside the MAXN×LDM buffers; skipped slabs are loaded and not used)
+        double an[PER_WAVE], bn[PER_WAVE];
+#pragma unroll
+        for (int u = 0; u < PER_WAVE; ++u) {
+            an[u] = am[(ti[u] * 16 + lr) * LD + lk];
+            bn[u] = bm[lk * LD + tj[u] * 16 + lr];
+        }
+#pragma unroll
+        for (int ks = 0; ks < MAXKS; ++ks) {
+            double ac[PER_WAVE], bc[PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < PER_WAVE; ++u) {
+                ac[u] = an[u];
+                bc[u] = bn[u];
+            }
+            if (ks + 1 < MAXKS) {
+#pragma unroll
+                for (int u = 0; u < PER_WAVE; ++u) {
+                    an[u] = am[(ti[u] * 16 + lr) * LD + 4 * (ks + 1) + lk];
+                    bn[u] = bm[(4 * (ks + 1) + lk) * LD + tj[u] * 16 + lr];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PER_WAVE; ++u)
+                if (mk[u] & (1u << ks)) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u], acc[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < PER_WAVE; ++u) {
+            if (wave + u * (NT / 64) >= T * T) continue;
+#pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const int r = ti * 16 + lk + 4 * j, c = tj * 16 + lr;
+                const int r = ti[u] * 16 + lk + 4 * j, c = tj[u] * 16 + lr;
                 const bool real = r < n && c < n;
-                double v = acc[j];
+                double v = acc[u][j];
                 if (make_t) {
                     const double d = (r == c) ? 1.0 : 0.0;
                     const double x = d - v;
@@ -316,7 +389,7 @@ struct IslandSolver {
             s.bcast[0] = 1.0 / (one * inf);
         }
         __syncthreads();
-        double sc = s.bcast[0];
+        double sc = uni(s.bcast[0]);
         double* Xm = X();
         for (int e = threadIdx.x; e < n * n; e += NT) {
             int row = e / n, col = e % n;
@@ -399,7 +472,7 @@ struct IslandSolver {
                 __syncthreads();
                 wnorm(0, 1);
                 __syncthreads();
-                rnorm = s.wn[0];
+                rnorm = uni(s.wn[0]);
             }
             if (rnorm < best) {
                 best = rnorm;
@@ -417,21 +490,27 @@ struct IslandSolver {
             matvec();
             __syncthreads();
             // all-zero correction → hand over to subdivision
-            if (threadIdx.x == 0) {
+            {
                 int nz = 0;
-                for (int t = 0; t < m * 6; ++t) nz |= (s.dv[t / 6][t % 6] != 0.0);
-                s.ibcast[0] = nz;
+                for (int t = threadIdx.x; t < m * 6; t += NT) nz |= (s.dv[t / 6][t % 6] != 0.0);
+                if (!__syncthreads_or(nz)) break;
             }
-            __syncthreads();
-            if (!s.ibcast[0]) break;
-            // 8-lane backtracking line search from the same base
-            for (int t = threadIdx.x; t < m * 6; t += NT) s.base[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
-            __syncthreads();
-            for (int t = threadIdx.x; t < LANES * m * 6; t += NT) {
-                int lane = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
-                double alpha = 1.0;
-                for (int q = 0; q < lane; ++q) alpha = alpha / 2.0;
-                s.vmid[lane][i][k] = s.base[i][k] + s.dv[i][k] * alpha;
+            // 8-lane backtracking line search from the same base, α = 2^-lane
+            {
+                double keep[(LANES * MAXM * 6 + NT - 1) / NT];
+                int u = 0;
+                for (int t = threadIdx.x; t < LANES * m * 6; t += NT, ++u) {
+                    int lane = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
+                    double b = s.vmid[0][i][k];
+                    keep[u] = b + s.dv[i][k] * ldexp(1.0, -lane);
+                    if (lane == 0) s.base[i][k] = b;
+                }
+                __syncthreads();
+                u = 0;
+                for (int t = threadIdx.x; t < LANES * m * 6; t += NT, ++u) {
+                    int lane = t / (m * 6), r = t % (m * 6);
+                    s.vmid[lane][r / 6][r % 6] = keep[u];
+                }
             }
             __syncthreads();
             pre(0, LANES);
@@ -445,7 +524,7 @@ struct IslandSolver {
             int accepted = -1;
             double tnorm = 0.0;
             for (int l = 0; l < LANES; ++l)
-                if (s.wn[l] < rnorm) { accepted = l; tnorm = s.wn[l]; break; }
+                if (uni(s.wn[l]) < rnorm) { accepted = l; tnorm = uni(s.wn[l]); break; }
             if (accepted >= 0) {
                 if (accepted > 0) {  // adopt_lane (cache.rs:1115-1136)
                     for (int t = threadIdx.x; t < m * 6; t += NT) {
@@ -660,6 +739,10 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
             }
         }
         s.inc_start[m] = o;
+        o = 0;
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < J; ++j)
+                if (s.jkind[j] == k) s.jorder[o++] = j;
         // A's non-zero 4-column slabs per 16-row tile, from the block neighbour lists
         for (int t = 0; t < IslandShared<MAXM, MAXJ>::MAXN / 16; ++t) s.kmask[t] = 0u;
         for (int bi = 0; bi < m; ++bi)

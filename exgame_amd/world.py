@@ -17,6 +17,8 @@ from . import _lib, pga
 INTEGRATORS = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2}
 # Newton–Schulz products: FP64 MFMA tiles, or scalar FMAs in the reference's block order
 NS_ENGINES = {"mfma": 0, "valu_ordered": 1}
+# f64 throughout, or FP32 arithmetic with two-sum compensated accumulation
+PRECISIONS = {"f64": 0, "f32c": 1}
 
 
 @dataclass(frozen=True)
@@ -28,7 +30,8 @@ class Epoch:
 
 class DeviceWorld:
     def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
-                 ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma"):
+                 ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
+                 jacobian_precision: str = "f64"):
         L = _lib.lib()
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
@@ -60,6 +63,7 @@ class DeviceWorld:
         self.h = h
         _lib.check(L.exg_world_set_solver(self.h, INTEGRATORS[integrator], float(tol2), int(ns_warm), int(ns_seed)))
         _lib.check(L.exg_world_set_ns_engine(self.h, NS_ENGINES[ns_engine]))
+        _lib.check(L.exg_world_set_precision(self.h, PRECISIONS[gravity_precision], PRECISIONS[jacobian_precision]))
 
     def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
         if isinstance(dt, Epoch):
@@ -110,9 +114,10 @@ class DeviceWorld:
             pass
 
 
-def gravity_accumulate(charges: np.ndarray, G: float = 1.0) -> np.ndarray:
+def gravity_accumulate(charges: np.ndarray, G: float = 1.0, precision: str = "f64") -> np.ndarray:
     ch = np.ascontiguousarray(charges, dtype=np.float64)
     n = ch.size // 8
     out = np.zeros((n, 6))
-    _lib.check(_lib.lib().exg_gravity_accumulate(n, _lib.fptr(ch), float(G), _lib.fptr(out)))
+    _lib.check(_lib.lib().exg_gravity_accumulate_precision(n, _lib.fptr(ch), float(G), PRECISIONS[precision],
+                                                           _lib.fptr(out)))
     return out

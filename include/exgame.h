@@ -194,6 +194,16 @@ int exg_fatals(exg_store* s, int kind, uint32_t* count);
  * charges[n*8] in, wrench[n*6] out (force, local × force), host pointers. */
 int exg_gravity_accumulate(int n, const double* charges, double G, double* wrench);
 
+/* Arithmetic of a path (no counterpart in the reference, whose kernels are f32-only,
+ * crates/newton/src/accelerator/mod.rs:1015-1019): EXG_PREC_F64 throughout, or
+ * EXG_PREC_F32_COMPENSATED — FP32 pair/element arithmetic with two-sum compensated
+ * FP32 accumulation, widened to f64 at the end; residuals, Newton–Schulz and the
+ * state update stay f64 either way. */
+enum { EXG_PREC_F64 = 0, EXG_PREC_F32_COMPENSATED = 1 };
+int exg_gravity_accumulate_precision(int n, const double* charges, double G, int precision, double* wrench);
+/* Per-world precision of gravity and of the Jacobian assembly. */
+int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
+
 #ifdef __cplusplus
 }
 #endif
