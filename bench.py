@@ -75,6 +75,23 @@ def cpu_baseline(target_s: float = 10.0):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def copy_calibration(nbytes: int) -> None:
+    """One device-to-device copy of `nbytes` through the HIP runtime (its blit kernel
+    reads and writes exactly nbytes), so HBM byte counters can be scaled."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)):
+        raise RuntimeError("copy calibration: hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+        raise RuntimeError("copy calibration: hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,13 +113,7 @@ def main() -> int:
     scene = scenes.jointed_chains(n_chains=CHAINS, m=M, seed=0)
     info = _lib.device_info()
     if args.copy_calibration_mib > 0:
-        import torch
-        n = args.copy_calibration_mib * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        copy_calibration(args.copy_calibration_mib << 20)
 
     # warm-up on a throwaway world (code objects, caches, clocks)
     if args.warmup > 0:

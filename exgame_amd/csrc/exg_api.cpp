@@ -57,6 +57,8 @@ struct exg_world {
     int ns_warm = 2, ns_seed = 24;
     int ns_engine = EXG_NS_MFMA;
     int max_m = 0, max_j = 0;
+    DevBuf<int> class_ids[exg::ISLAND_CLASSES];   // island ids routed to each kernel variant
+    int class_n[exg::ISLAND_CLASSES] = {};
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
     DevBuf<float> grav_narrow;
@@ -77,6 +79,7 @@ struct exg_world {
         for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
         grav_narrow.release();
         xacc_off.release();
+        for (auto& c : class_ids) c.release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
         if (stream) (void)hipStreamDestroy(stream);
@@ -184,6 +187,19 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(w->seeded.alloc(I));
     W_TRY(w->stats.alloc(I * exg::ISLAND_STATS));
     W_TRY(w->xacc_off.alloc(I));
+    {
+        std::vector<int> ids[exg::ISLAND_CLASSES];
+        for (size_t i = 0; i < I; ++i) {
+            int c = exg::island_class(island_start[i + 1] - island_start[i], jstart[i + 1] - jstart[i]);
+            ids[c].push_back((int)i);
+        }
+        for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
+            w->class_n[c] = (int)ids[c].size();
+            if (ids[c].empty()) continue;
+            W_TRY(w->class_ids[c].alloc(ids[c].size()));
+            W_TRY(hipMemcpy(w->class_ids[c].p, ids[c].data(), ids[c].size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
     std::vector<long long> xoff(I);
     long long tot = 0;
     for (size_t i = 0; i < I; ++i) {
@@ -325,8 +341,17 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         a.ns_seed = w->ns_seed;
         a.ns_engine = w->ns_engine;
         a.n_islands = w->I;
-        if (w->integrator == EXG_NEWTON) exg::launch_island_step(a, w->max_m, w->max_j, w->stream);
-        else exg::launch_explicit_step(a, w->integrator, w->stream);
+        if (w->integrator == EXG_NEWTON) {
+            for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
+                if (!w->class_n[c]) continue;
+                a.island_ids = w->class_ids[c].p;
+                a.n_islands = w->class_n[c];
+                exg::launch_island_class(a, c, w->stream);
+            }
+        } else {
+            a.island_ids = nullptr;
+            exg::launch_explicit_step(a, w->integrator, w->stream);
+        }
         HIP_TRY(hipGetLastError());
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
         w->front = 1 - w->front;

@@ -33,9 +33,14 @@ struct IslandArgs {
     double dt, warp, tol2, G;
     int ns_warm, ns_seed, n_islands;
     int ns_engine;            // 0: Newton–Schulz products on MFMA; 1: VALU in the oracle's summation order
+    const int* island_ids;    // islands this launch covers (n_islands of them); null: 0 .. n_islands-1
 };
 
-void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream);
+// Island classes of the fused Newton kernel and the launch of one class over the
+// island ids in a.island_ids[0 .. a.n_islands).
+enum { ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASSES = 4 };
+int island_class(int m, int joints);   // -1 when no variant holds the island
+void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream);
 
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);

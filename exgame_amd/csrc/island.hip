@@ -39,43 +39,64 @@ __device__ inline double uni(double v) {
 }
 
 constexpr int NT = ISLAND_THREADS;
-constexpr int LANES = 8;
+constexpr int LANES = 8;   // line-search step lengths α = 2^-l, l < LANES (newton.rs:469-522)
+constexpr int LS = 4;      // candidates evaluated side by side: lanes 0-3, then 4-7 if all four fail
 constexpr int MAX_ITERATIONS = 32;
 constexpr int MAX_DEPTH = 8;
 constexpr int THRESHOLD_LOG2 = 8;
 constexpr int PER_HALVING_LOG2 = 4;
 
+// Line-search candidates 1..LS-1 (candidate 0 is the Newton iterate's own storage).
+template <int MAXM, int MAXJ>
+struct LaneSlots {
+    static constexpr int JD = MAXJ > 0 ? MAXJ : 1;
+    double vmid[LS - 1][MAXM][6];
+    double mid[LS - 1][MAXM][8];
+    double total[LS - 1][MAXM][6];
+    double rhs[LS - 1][MAXM][6];
+    double scale[LS - 1][MAXM][2];
+    double ratio[LS - 1][MAXM][2];
+    double conn[LS - 1][JD][2][6];
+};
+
 // The n×n system matrices (n = 6m) are stored flat, row-major, padded to n16 =
 // ⌈n/16⌉·16 rows and columns for the 16×16 MFMA tiles, with row stride LDM =
 // MAXN + 1 (odd in doubles): the sixteen rows an MFMA operand column spans then fall
 // on sixteen different LDS bank pairs. The padding stays zero.
-template <int MAXM, int MAXJ>
+//
+// Three things are never live at once and share one region: the joint Jacobians
+// (evaluation → assembly), T = 2I − A·X (inside one Newton–Schulz iteration) and
+// line-search candidates 1..3 (line search → adoption). With MAXJ = 14 an 8-body
+// island fits in 80 KB, so two islands share a CU.
+template <int MAXM, int MAXJ, int NTH>
 struct IslandShared {
     static constexpr int MAXN = ((6 * MAXM + 15) / 16) * 16;
     static constexpr int LDM = MAXN + 1;
     static constexpr int NB = MAXN * LDM;
-    double mat[4][NB];                      // A, X, X', R (X and X' swap by index)
+    static constexpr int JD = MAXJ > 0 ? MAXJ : 1;   // array extent (islands without joints: MAXJ = 0)
+    double A[NB];
+    double X[NB];
+    union {
+        double T[NB];
+        double jac[JD][2][24][6];           // ∂wrench[end]/∂axis, per joint
+        LaneSlots<MAXM, MAXJ> ls;
+    } u;
     unsigned kmask[MAXN / 16];              // per 16-row tile of A: 4-column slabs that can be non-zero
-    double jac[MAXJ][2][24][6];             // ∂wrench[end]/∂axis, per joint
-    double vmid[LANES][MAXM][6];
-    double mid[LANES][MAXM][8];
-    double total[LANES][MAXM][6];
-    double rhs[LANES][MAXM][6];
-    double scale[LANES][MAXM][2];
-    double conn[LANES][MAXJ][2][6];
-    double wn[LANES];
-    double ratio[LANES][MAXM][2];
-    double mass0[MAXM][36];                 // lane 0's world mass block
+    double vmid0[MAXM][6], mid0[MAXM][8], total0[MAXM][6], rhs0[MAXM][6], scale0[MAXM][2], ratio0[MAXM][2];
+    double conn0[JD][2][6];
+    double wn[LS];
+    double mass0[MAXM][36];                 // candidate 0's world mass block
     double pose[MAXM][8], mom[MAXM][6], massv[MAXM], tens[MAXM][9], ext[MAXM][6];
     double snap[MAXM][6], dv[MAXM][6], base[MAXM][6], best[MAXM][6];
-    double jparams[MAXJ][10];
-    double red[NT];
+    double jparams[JD][10];
+    double red[NTH / 64];                    // wave partial sums of block_sum
+    double seedsum[2 * MAXM];               // seed_transpose: block-row and block-column |A| sums
     double bcast[8];
-    int jkind[MAXJ], ja[MAXJ], jb[MAXJ];
-    int jorder[MAXJ];                       // joints grouped by kind (thread mapping only)
+    int jkind[JD], ja[JD], jb[JD];
+    int jorder[JD];                         // joints grouped by kind (thread mapping only)
     int nbr_start[MAXM + 1], nbr[MAXM * MAXM];
-    int term_start[MAXM * MAXM + 1], terms[4 * MAXJ];
-    int inc_start[MAXM + 1], inc[2 * MAXJ];
+    int term_start[MAXM * MAXM + 1], terms[4 * JD];
+    int inc_start[MAXM + 1], inc[2 * JD];
     int ibcast[8];
 };
 
@@ -83,8 +104,8 @@ struct IslandShared {
 // then the four wave sums added in wave order by every thread (the consumers compare
 // the result against a ×4 margin or take ⌊log₂⌋, never test it for equality). The
 // leading barrier protects s.red from the previous call's readers.
-template <int MAXM, int MAXJ>
-__device__ double block_sum(IslandShared<MAXM, MAXJ>& s, double v) {
+template <int MAXM, int MAXJ, int NTH>
+__device__ double block_sum(IslandShared<MAXM, MAXJ, NTH>& s, double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
@@ -92,72 +113,81 @@ __device__ double block_sum(IslandShared<MAXM, MAXJ>& s, double v) {
     __syncthreads();
     double r = s.red[0];
 #pragma unroll
-    for (int w = 1; w < NT / 64; ++w) r = r + s.red[w];
+    for (int w = 1; w < NTH / 64; ++w) r = r + s.red[w];
     return uni(r);
 }
 
-template <int MAXM, int MAXJ>
+template <int MAXM, int MAXJ, int NTH>
 struct IslandSolver {
-    IslandShared<MAXM, MAXJ>& s;
+    IslandShared<MAXM, MAXJ, NTH>& s;
     const IslandArgs& a;
-    static constexpr int LD = IslandShared<MAXM, MAXJ>::LDM;
+    static constexpr int LD = IslandShared<MAXM, MAXJ, NTH>::LDM;
     int m, J, isl, first_body;
     int n, n16;               // system size 6m and its 16-padded size
     bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
-    int ix, ixn;              // which of mat[1], mat[2] is X and which X'
     double dt, warp, half, span;
     long long xoff;
     int st_spans, st_newton, st_ns, st_reseed, st_reject, st_uncommit;
     bool seeded;
 
-    __device__ double* A() { return s.mat[0]; }
-    __device__ double* X() { return s.mat[ix]; }
-    __device__ double* Xn() { return s.mat[ixn]; }
-    __device__ double* R() { return s.mat[3]; }
+    __device__ double* A() { return s.A; }
+    __device__ double* X() { return s.X; }
+    __device__ double* R() { return s.u.T; }
+    // candidate l's arrays (l = 0 is the Newton iterate)
+    static __device__ int sl(int l) { return l > 0 ? l - 1 : 0; }
+    __device__ double* VMID(int l, int i) { return l == 0 ? s.vmid0[i] : s.u.ls.vmid[sl(l)][i]; }
+    __device__ double* MID(int l, int i) { return l == 0 ? s.mid0[i] : s.u.ls.mid[sl(l)][i]; }
+    __device__ double* TOTAL(int l, int i) { return l == 0 ? s.total0[i] : s.u.ls.total[sl(l)][i]; }
+    __device__ double* RHS(int l, int i) { return l == 0 ? s.rhs0[i] : s.u.ls.rhs[sl(l)][i]; }
+    __device__ double* SCALE(int l, int i) { return l == 0 ? s.scale0[i] : s.u.ls.scale[sl(l)][i]; }
+    __device__ double* RATIO(int l, int i) { return l == 0 ? s.ratio0[i] : s.u.ls.ratio[sl(l)][i]; }
+    __device__ double* CONN(int l, int j, int e) { return l == 0 ? s.conn0[j][e] : s.u.ls.conn[sl(l)][j][e]; }
 
     // ───── PRE: midpoint = pose ∘ exp(half·vmid) for lanes [l0, l1) ─────
     __device__ void pre(int l0, int l1) {
-        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NT) {
+        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NTH) {
             int lane = l0 + t / m, i = t % m;
             double e[8];
-            twist_exp(s.vmid[lane][i], half, e);
-            compose(s.pose[i], e, s.mid[lane][i]);
+            twist_exp(VMID(lane, i), half, e);
+            compose(s.pose[i], e, MID(lane, i));
         }
     }
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
     __device__ void joints_jacobian() {
+        if constexpr (MAXJ == 0) return;
         using D = Dual<double>;
-        for (int t = threadIdx.x; t < J * 24; t += NT) {
+        for (int t = threadIdx.x; t < J * 24; t += NTH) {
             int j = s.jorder[t / 24], ax = t % 24;
             int ends[2] = {s.ja[j], s.jb[j]};
             D pp[2][8], vv[2][6];
             for (int e = 0; e < 2; ++e) {
-                const double* base = s.mid[0][ends[e]];
+                const double* base = s.mid0[ends[e]];
                 if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
                 else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-                const double* v = s.vmid[0][ends[e]];
+                const double* v = s.vmid0[ends[e]];
                 for (int k = 0; k < 6; ++k) vv[e][k] = D(v[k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
             }
             D wa[6], wb[6];
             joint_wrench(s.jkind[j], s.jparams[j], pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
             for (int k = 0; k < 6; ++k) {
-                s.jac[j][0][ax][k] = wa[k].d;
-                s.jac[j][1][ax][k] = wb[k].d;
+                s.u.jac[j][0][ax][k] = wa[k].d;
+                s.u.jac[j][1][ax][k] = wb[k].d;
             }
             if (ax == 0)
                 for (int k = 0; k < 6; ++k) {
-                    s.conn[0][j][0][k] = wa[k].v;
-                    s.conn[0][j][1][k] = wb[k].v;
+                    s.conn0[j][0][k] = wa[k].v;
+                    s.conn0[j][1][k] = wb[k].v;
                 }
         }
     }
     // ───── value-only joints for lanes [l0, l1) ─────
     __device__ void joints_plain(int l0, int l1) {
+        if constexpr (MAXJ == 0) return;
         const int nl = l1 - l0;
-        for (int t = threadIdx.x; t < nl * J; t += NT) {
+        for (int t = threadIdx.x; t < nl * J; t += NTH) {
             int lane = l0 + t % nl, j = s.jorder[t / nl];
-            joint_wrench(s.jkind[j], s.jparams[j], s.mid[lane][s.ja[j]], s.mid[lane][s.jb[j]], s.vmid[lane][s.ja[j]],
-                         s.vmid[lane][s.jb[j]], dt, warp, s.conn[lane][j][0], s.conn[lane][j][1]);
+            joint_wrench(s.jkind[j], s.jparams[j], MID(lane, s.ja[j]), MID(lane, s.jb[j]), VMID(lane, s.ja[j]),
+                         VMID(lane, s.jb[j]), dt, warp, CONN(lane, j, 0), CONN(lane, j, 1));
         }
     }
     // ───── gather (external + incident wrenches in joint order) + body post ─────
@@ -166,20 +196,21 @@ struct IslandSolver {
         for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
         for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
             int code = s.inc[q];
-            const double* w = s.conn[lane][code >> 1][code & 1];
+            const double* w = CONN(lane, code >> 1, code & 1);
             for (int k = 0; k < 6; ++k) tot[k] = tot[k] + w[k];
         }
-        for (int k = 0; k < 6; ++k) s.total[lane][i][k] = tot[k];
+        double* tl = TOTAL(lane, i);
+        for (int k = 0; k < 6; ++k) tl[k] = tot[k];
         double block[6][6];
         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-        body_post(s.mid[lane][i], s.vmid[lane][i], s.massv[i], s.tens[i], s.snap[i], tot, half, floor2, block,
-                  s.rhs[lane][i], s.scale[lane][i]);
+        body_post(MID(lane, i), VMID(lane, i), s.massv[i], s.tens[i], s.snap[i], tot, half, floor2, block,
+                  RHS(lane, i), SCALE(lane, i));
         if (write_mass)
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = block[r][c];
     }
     __device__ void gather_post(int l0, int l1) {
-        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NT) {
+        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NTH) {
             int lane = l0 + t / m, i = t % m;
             gather_post_one(lane, i, lane == 0);
         }
@@ -191,20 +222,23 @@ struct IslandSolver {
     // order with the same max2 (so NaN handling matches the sequential form).
     __device__ void wnorm(int l0, int l1) {
         const int nl = l1 - l0;
-        for (int t = threadIdx.x; t < nl * m; t += NT) {
+        for (int t = threadIdx.x; t < nl * m; t += NTH) {
             const int lane = l0 + t / m, i = t % m;
-            const double* r = s.rhs[lane][i];
+            const double* r = RHS(lane, i);
+            const double* sc = SCALE(lane, i);
             double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
             double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
-            s.ratio[lane][i][0] = lin2 / s.scale[lane][i][0];
-            s.ratio[lane][i][1] = ang2 / s.scale[lane][i][1];
+            double* ra = RATIO(lane, i);
+            ra[0] = lin2 / sc[0];
+            ra[1] = ang2 / sc[1];
         }
         __syncthreads();
-        for (int lane = l0 + (int)threadIdx.x; lane < l1; lane += NT) {
+        for (int lane = l0 + (int)threadIdx.x; lane < l1; lane += NTH) {
             double worst = 0.0;
             for (int i = 0; i < m; ++i) {
-                worst = max2(worst, s.ratio[lane][i][0]);
-                worst = max2(worst, s.ratio[lane][i][1]);
+                const double* ra = RATIO(lane, i);
+                worst = max2(worst, ra[0]);
+                worst = max2(worst, ra[1]);
             }
             s.wn[lane] = worst;
         }
@@ -213,7 +247,7 @@ struct IslandSolver {
     __device__ void assemble() {
         double pf = -(half * half), vf = -half;
         double* Am = A();
-        for (int e = threadIdx.x; e < m * m * 6; e += NT) {
+        for (int e = threadIdx.x; e < m * m * 6; e += NTH) {
             const int b = e / 6, r = e - 6 * b, i = b / m, j = b - i * m;
             const int q0 = s.term_start[b], q1 = s.term_start[b + 1];
             const double dg = (i == j) ? 1.0 : 0.0;
@@ -226,8 +260,8 @@ struct IslandSolver {
                 const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    double p = s.jac[jn][re][ce * 6 + c][r];
-                    double v = s.jac[jn][re][12 + ce * 6 + c][r];
+                    double p = s.u.jac[jn][re][ce * 6 + c][r];
+                    double v = s.u.jac[jn][re][12 + ce * 6 + c][r];
                     acc[c] = acc[c] + pf * p + vf * v;
                 }
             }
@@ -236,9 +270,14 @@ struct IslandSolver {
         }
     }
     // ───── VALU form, the oracle's block order: out = 0·out + Σ_k a[i,k]·(sign·b[k,j] + diag·2I) ─────
-    __device__ void gemm(const double* am, const double* bm, double* out, bool two_minus, bool sparse) {
+    // With in_place, every cell is computed before any is stored (out may be bm).
+    __device__ void gemm(const double* am, const double* bm, double* out, bool two_minus, bool sparse,
+                         bool in_place) {
+        constexpr int MAXCELLS = (IslandShared<MAXM, MAXJ, NTH>::MAXN * IslandShared<MAXM, MAXJ, NTH>::MAXN + NTH - 1) / NTH;
+        double cell[MAXCELLS];
         double sign = two_minus ? -1.0 : 1.0;
-        for (int e = threadIdx.x; e < n * n; e += NT) {
+        int u = 0;
+        for (int e = threadIdx.x; e < n * n; e += NTH, ++u) {
             int row = e / n, col = e % n, i = row / 6, j = col / 6, c = col % 6;
             double acc = 0.0 * out[row * LD + col];
             int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : m;
@@ -255,8 +294,11 @@ struct IslandSolver {
                 }
                 acc = acc + sum;
             }
-            out[row * LD + col] = acc;
+            cell[u] = acc;
         }
+        if (in_place) __syncthreads();
+        u = 0;
+        for (int e = threadIdx.x; e < n * n; e += NTH, ++u) out[(e / n) * LD + e % n] = cell[u];
     }
     // ───── MFMA form: out = a·b over 16×16 tiles, k in 4-column slabs ─────
     // Wave w computes tiles w, w+4, … of the n16×n16 product; a slab whose bit is
@@ -265,15 +307,16 @@ struct IslandSolver {
     // returns this thread's share of ‖I − a·b‖². Padding rows/columns are stored 0.
     // Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l supplies
     // a[l%16][l/16] and b[l/16][l%16]; accumulator j holds c[l/16 + 4j][l%16].
+    // With in_place, all tiles are multiplied before any is stored (out may be am).
     __device__ double mfma_product(const double* am, const double* bm, double* out, const unsigned* kmask,
-                                   bool make_t) {
+                                   bool make_t, bool in_place) {
         const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
         const unsigned dense = (1u << KS) - 1u;
-        constexpr int MAXKS = IslandShared<MAXM, MAXJ>::MAXN / 4;
-        constexpr int MAXT = IslandShared<MAXM, MAXJ>::MAXN / 16;
-        constexpr int PER_WAVE = (MAXT * MAXT + NT / 64 - 1) / (NT / 64);
+        constexpr int MAXKS = IslandShared<MAXM, MAXJ, NTH>::MAXN / 4;
+        constexpr int MAXT = IslandShared<MAXM, MAXJ, NTH>::MAXN / 16;
+        constexpr int PER_WAVE = (MAXT * MAXT + NTH / 64 - 1) / (NTH / 64);
         double part = 0.0;
         // this wave's tiles: wave, wave + 4, … (wave-uniform), accumulated side by side
         int ti[PER_WAVE], tj[PER_WAVE];
@@ -281,7 +324,7 @@ struct IslandSolver {
         d4 acc[PER_WAVE];
 #pragma unroll
         for (int u = 0; u < PER_WAVE; ++u) {
-            const int t = wave + u * (NT / 64);
+            const int t = wave + u * (NTH / 64);
             const bool live = t < T * T;
             ti[u] = live ? t / T : 0;
             tj[u] = live ? t % T : 0;
@@ -315,9 +358,10 @@ struct IslandSolver {
             for (int u = 0; u < PER_WAVE; ++u)
                 if (mk[u] & (1u << ks)) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u], acc[u], 0, 0, 0);
         }
+        if (in_place) __syncthreads();
 #pragma unroll
         for (int u = 0; u < PER_WAVE; ++u) {
-            if (wave + u * (NT / 64) >= T * T) continue;
+            if (wave + u * (NTH / 64) >= T * T) continue;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = ti[u] * 16 + lk + 4 * j, c = tj[u] * 16 + lr;
@@ -337,7 +381,7 @@ struct IslandSolver {
     // ───── Σ ‖diag·I − b‖² over the n×n entries of `mat` ─────
     __device__ double reduce_sq(const double* mat, bool identity_diag) {
         double part = 0.0;
-        for (int e = threadIdx.x; e < n * n; e += NT) {
+        for (int e = threadIdx.x; e < n * n; e += NTH) {
             int row = e / n, col = e % n;
             double diag = (identity_diag && row == col) ? 1.0 : 0.0;
             double v = mat[row * LD + col];
@@ -349,20 +393,17 @@ struct IslandSolver {
     __device__ double refine_inverse() {  // newton.rs:255-279
         double before;
         if (mfma) {
-            // R ← T = 2I − A·X (A block-sparse: zero slabs skipped), then X' = X·T
-            double part = mfma_product(A(), X(), R(), s.kmask, true);
+            // T = 2I − A·X (A block-sparse: zero slabs skipped), then X ← X·T in place
+            double part = mfma_product(A(), X(), R(), s.kmask, true, false);
             before = block_sum(s, part);  // its barrier also publishes T
-            mfma_product(X(), R(), Xn(), nullptr, false);
+            mfma_product(X(), R(), X(), nullptr, false, true);
         } else {
-            gemm(A(), X(), R(), false, true);
+            gemm(A(), X(), R(), false, true, false);
             __syncthreads();
             before = reduce_sq(R(), true);
-            gemm(X(), R(), Xn(), true, false);
+            gemm(X(), R(), X(), true, false, true);
         }
         __syncthreads();
-        int t = ix;
-        ix = ixn;
-        ixn = t;
         st_ns++;
         return before;
     }
@@ -377,21 +418,21 @@ struct IslandSolver {
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c)
                         sum += fabs(col ? Am[(6 * j + r) * LD + 6 * i + c] : Am[(6 * i + r) * LD + 6 * j + c]);
-            s.red[threadIdx.x] = sum;
+            s.seedsum[threadIdx.x] = sum;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
             double one = 0.0, inf = 0.0;
             for (int i = 0; i < m; ++i) {
-                inf = max2(inf, s.red[i]);
-                one = max2(one, s.red[m + i]);
+                inf = max2(inf, s.seedsum[i]);
+                one = max2(one, s.seedsum[m + i]);
             }
             s.bcast[0] = 1.0 / (one * inf);
         }
         __syncthreads();
         double sc = uni(s.bcast[0]);
         double* Xm = X();
-        for (int e = threadIdx.x; e < n * n; e += NT) {
+        for (int e = threadIdx.x; e < n * n; e += NTH) {
             int row = e / n, col = e % n;
             Xm[row * LD + col] = A()[col * LD + row] * sc;
         }
@@ -420,35 +461,32 @@ struct IslandSolver {
     // ───── dv = 0·dv + Σ_j X[i,j]·rhs_j ─────
     __device__ void matvec() {
         const double* Xm = X();
-        for (int t = threadIdx.x; t < n; t += NT) {
+        for (int t = threadIdx.x; t < n; t += NTH) {
             int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
             for (int j = 0; j < m; ++j) {
                 const double* xb = Xm + t * LD + 6 * j;
                 double sum = 0.0;
-                for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.rhs[0][j][c];
+                for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.rhs0[j][c];
                 acc = acc + sum;
             }
             s.dv[i][r] = acc;
         }
     }
     __device__ void copy_vmid(int dst, const double (*src)[6]) {
-        for (int t = threadIdx.x; t < m * 6; t += NT) s.vmid[dst][t / 6][t % 6] = src[t / 6][t % 6];
+        for (int t = threadIdx.x; t < m * 6; t += NTH) VMID(dst, t / 6)[t % 6] = src[t / 6][t % 6];
     }
 
     // ───── one Newton solve at the current `half` (newton.rs:334-530) ─────
     __device__ void newton_solve(double tol2, int ns_warm, int ns_seed, bool& converged, double& best,
                                  bool& assembled) {
         const double huge = 1000000000.0;
-        // lane 0 iterate = world velocity of each body; the other lanes start from it
-        for (int t = threadIdx.x; t < m; t += NT)
-            world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.vmid[0][t]);
+        // the iterate starts at each body's world velocity (the line-search candidates
+        // are written from it before they are read)
+        for (int t = threadIdx.x; t < m; t += NTH)
+            world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.vmid0[t]);
         __syncthreads();
-        for (int t = threadIdx.x; t < (LANES - 1) * m * 6; t += NT) {
-            int lane = 1 + t / (m * 6), r = t % (m * 6);
-            s.vmid[lane][r / 6][r % 6] = s.vmid[0][r / 6][r % 6];
-        }
-        for (int t = threadIdx.x; t < m * 6; t += NT) s.best[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
+        for (int t = threadIdx.x; t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
         __syncthreads();
         best = huge;
         converged = false;
@@ -476,7 +514,7 @@ struct IslandSolver {
             }
             if (rnorm < best) {
                 best = rnorm;
-                for (int t = threadIdx.x; t < m * 6; t += NT) s.best[t / 6][t % 6] = s.vmid[0][t / 6][t % 6];
+                for (int t = threadIdx.x; t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
             }
             if (rnorm <= tol2) { converged = true; break; }
             if (!(rnorm < huge)) break;
@@ -492,52 +530,52 @@ struct IslandSolver {
             // all-zero correction → hand over to subdivision
             {
                 int nz = 0;
-                for (int t = threadIdx.x; t < m * 6; t += NT) nz |= (s.dv[t / 6][t % 6] != 0.0);
+                for (int t = threadIdx.x; t < m * 6; t += NTH) nz |= (s.dv[t / 6][t % 6] != 0.0);
                 if (!__syncthreads_or(nz)) break;
             }
-            // 8-lane backtracking line search from the same base, α = 2^-lane
-            {
-                double keep[(LANES * MAXM * 6 + NT - 1) / NT];
-                int u = 0;
-                for (int t = threadIdx.x; t < LANES * m * 6; t += NT, ++u) {
-                    int lane = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
-                    double b = s.vmid[0][i][k];
-                    keep[u] = b + s.dv[i][k] * ldexp(1.0, -lane);
-                    if (lane == 0) s.base[i][k] = b;
+            // backtracking line search from the same base, α = 2^-lane for lanes 0..7,
+            // evaluated LS at a time; the first lane (in order) that lowers the
+            // residual is taken, as when all eight are evaluated together
+            for (int t = threadIdx.x; t < m * 6; t += NTH) s.base[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+            __syncthreads();
+            int accepted = -1, accepted_slot = -1;
+            double tnorm = 0.0;
+            for (int round = 0; round < LANES / LS && accepted < 0; ++round) {
+                for (int t = threadIdx.x; t < LS * m * 6; t += NTH) {
+                    int slot = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
+                    VMID(slot, i)[k] = s.base[i][k] + s.dv[i][k] * ldexp(1.0, -(round * LS + slot));
                 }
                 __syncthreads();
-                u = 0;
-                for (int t = threadIdx.x; t < LANES * m * 6; t += NT, ++u) {
-                    int lane = t / (m * 6), r = t % (m * 6);
-                    s.vmid[lane][r / 6][r % 6] = keep[u];
-                }
-            }
-            __syncthreads();
-            pre(0, LANES);
-            __syncthreads();
-            joints_plain(0, LANES);
-            __syncthreads();
-            gather_post(0, LANES);
-            __syncthreads();
-            wnorm(0, LANES);
-            __syncthreads();
-            int accepted = -1;
-            double tnorm = 0.0;
-            for (int l = 0; l < LANES; ++l)
-                if (uni(s.wn[l]) < rnorm) { accepted = l; tnorm = uni(s.wn[l]); break; }
-            if (accepted >= 0) {
-                if (accepted > 0) {  // adopt_lane (cache.rs:1115-1136)
-                    for (int t = threadIdx.x; t < m * 6; t += NT) {
-                        s.vmid[0][t / 6][t % 6] = s.vmid[accepted][t / 6][t % 6];
-                        s.rhs[0][t / 6][t % 6] = s.rhs[accepted][t / 6][t % 6];
+                pre(0, LS);
+                __syncthreads();
+                joints_plain(0, LS);
+                __syncthreads();
+                gather_post(0, LS);
+                __syncthreads();
+                wnorm(0, LS);
+                __syncthreads();
+                for (int l = 0; l < LS; ++l)
+                    if (uni(s.wn[l]) < rnorm) {
+                        accepted = round * LS + l;
+                        accepted_slot = l;
+                        tnorm = uni(s.wn[l]);
+                        break;
                     }
-                    for (int t = threadIdx.x; t < m * 8; t += NT) s.mid[0][t / 8][t % 8] = s.mid[accepted][t / 8][t % 8];
+            }
+            if (accepted >= 0) {
+                if (accepted_slot > 0) {  // adopt_lane (cache.rs:1115-1136)
+                    const int as = accepted_slot;
+                    for (int t = threadIdx.x; t < m * 6; t += NTH) {
+                        s.vmid0[t / 6][t % 6] = VMID(as, t / 6)[t % 6];
+                        s.rhs0[t / 6][t % 6] = RHS(as, t / 6)[t % 6];
+                    }
+                    for (int t = threadIdx.x; t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
                     // the accepted lane's mass block, recomputed bit-identically from its inputs
-                    for (int t = threadIdx.x; t < m; t += NT) {
+                    for (int t = threadIdx.x; t < m; t += NTH) {
                         double block[6][6], rr[6], sc[2];
                         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-                        body_post(s.mid[accepted][t], s.vmid[accepted][t], s.massv[t], s.tens[t], s.snap[t],
-                                  s.total[accepted][t], half, floor2, block, rr, sc);
+                        body_post(MID(as, t), VMID(as, t), s.massv[t], s.tens[t], s.snap[t], TOTAL(as, t), half, floor2,
+                                  block, rr, sc);
                         for (int r = 0; r < 6; ++r)
                             for (int c = 0; c < 6; ++c) s.mass0[t][r * 6 + c] = block[r][c];
                     }
@@ -546,7 +584,7 @@ struct IslandSolver {
                 has_carried = true;
                 carried = tnorm;
             } else {
-                for (int t = threadIdx.x; t < m * 6; t += NT) s.vmid[0][t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
+                for (int t = threadIdx.x; t < m * 6; t += NTH) s.vmid0[t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
                 __syncthreads();
                 st_reject++;
                 break;
@@ -578,7 +616,7 @@ struct IslandSolver {
         double mn = 0.0;
         {
             double part = 0.0;
-            for (int e = threadIdx.x; e < m * 36; e += NT) {
+            for (int e = threadIdx.x; e < m * 36; e += NTH) {
                 double v = s.mass0[e / 36][e % 36];
                 part = part + v * v;
             }
@@ -590,13 +628,13 @@ struct IslandSolver {
     __device__ void restore_x() {
         double* Xm = X();
         const double* src = a.xacc + xoff;
-        for (int e = threadIdx.x; e < n * n; e += NT) Xm[(e / n) * LD + e % n] = src[e];
+        for (int e = threadIdx.x; e < n * n; e += NTH) Xm[(e / n) * LD + e % n] = src[e];
         __syncthreads();
     }
     __device__ void publish_x(bool& accepted_seeded) {
         const double* Xm = X();
         double* dst = a.xacc + xoff;
-        for (int e = threadIdx.x; e < n * n; e += NT) dst[e] = Xm[(e / n) * LD + e % n];
+        for (int e = threadIdx.x; e < n * n; e += NTH) dst[e] = Xm[(e / n) * LD + e % n];
         double nx = reduce_sq(Xm, false);
         const double huge = 1000000000.0;
         if (nx < huge * huge) accepted_seeded = seeded;
@@ -616,7 +654,7 @@ struct IslandSolver {
             for (int k = 0; k < depth; ++k) span = span / 2.0;
             half = span / 2.0;
             st_spans++;
-            for (int t = threadIdx.x; t < m; t += NT) world_momentum(s.pose[t], s.mom[t], s.snap[t]);
+            for (int t = threadIdx.x; t < m; t += NTH) world_momentum(s.pose[t], s.mom[t], s.snap[t]);
             __syncthreads();
             bool converged, assembled;
             double best;
@@ -636,8 +674,8 @@ struct IslandSolver {
             }
             if (converged || finite) {
                 if (!converged) st_uncommit++;
-                for (int t = threadIdx.x; t < m; t += NT)
-                    commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.vmid[0][t], s.snap[t], span, half,
+                for (int t = threadIdx.x; t < m; t += NTH)
+                    commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.vmid0[t], s.snap[t], span, half,
                                 converged);
                 __syncthreads();
                 publish_x(accepted_seeded);
@@ -660,41 +698,39 @@ struct IslandSolver {
     }
 };
 
-template <int MAXM, int MAXJ>
-__global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
-    __shared__ IslandShared<MAXM, MAXJ> s;
-    const int isl = blockIdx.x;
-    if (isl >= a.n_islands) return;
+template <int MAXM, int MAXJ, int NTH>
+__device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH>& s) {
+    if ((int)blockIdx.x >= a.n_islands) return;
+    const int isl = a.island_ids ? a.island_ids[blockIdx.x] : (int)blockIdx.x;
     const int b0 = a.island_start[isl], m = a.island_start[isl + 1] - b0;
     const int j0 = a.island_jstart[isl], J = a.island_jstart[isl + 1] - j0;
     if (m > MAXM || J > MAXJ || m <= 0) return;  // the host routes islands by size; never taken
 
     // ── load the island ──
-    for (int t = threadIdx.x; t < m * 8; t += NT) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
-    for (int t = threadIdx.x; t < m * 6; t += NT) {
+    for (int t = threadIdx.x; t < m * 8; t += NTH) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
+    for (int t = threadIdx.x; t < m * 6; t += NTH) {
         s.mom[t / 6][t % 6] = a.mom[(size_t)(b0 + t / 6) * 6 + t % 6];
         s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
         s.dv[t / 6][t % 6] = 0.0;
     }
-    for (int t = threadIdx.x; t < m * 9; t += NT) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];
-    for (int t = threadIdx.x; t < m; t += NT) s.massv[t] = a.mass[b0 + t];
-    for (int t = threadIdx.x; t < J; t += NT) {
+    for (int t = threadIdx.x; t < m * 9; t += NTH) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];
+    for (int t = threadIdx.x; t < m; t += NTH) s.massv[t] = a.mass[b0 + t];
+    for (int t = threadIdx.x; t < J; t += NTH) {
         s.jkind[t] = a.jkind[j0 + t];
         s.ja[t] = a.ja[j0 + t] - b0;
         s.jb[t] = a.jb[j0 + t] - b0;
         for (int p = 0; p < 10; ++p) s.jparams[t][p] = a.jparams[(size_t)(j0 + t) * 10 + p];
     }
     const long long xoff = a.xacc_off[isl];
-    constexpr int LD = IslandShared<MAXM, MAXJ>::LDM;
+    constexpr int LD = IslandShared<MAXM, MAXJ, NTH>::LDM;
     const int n = 6 * m;
-    for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ>::NB; e += NT) {
-        s.mat[0][e] = 0.0;
-        s.mat[1][e] = 0.0;
-        s.mat[2][e] = 0.0;
-        s.mat[3][e] = 0.0;
+    for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ, NTH>::NB; e += NTH) {
+        s.A[e] = 0.0;
+        s.X[e] = 0.0;
+        s.u.T[e] = 0.0;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < n * n; e += NT) s.mat[1][(e / n) * LD + e % n] = a.xacc[xoff + e];
+    for (int e = threadIdx.x; e < n * n; e += NTH) s.X[(e / n) * LD + e % n] = a.xacc[xoff + e];
     __syncthreads();
     // ── topology tables: neighbours, assembly terms, incidence (joint order) ──
     if (threadIdx.x == 0) {
@@ -744,7 +780,7 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
             for (int j = 0; j < J; ++j)
                 if (s.jkind[j] == k) s.jorder[o++] = j;
         // A's non-zero 4-column slabs per 16-row tile, from the block neighbour lists
-        for (int t = 0; t < IslandShared<MAXM, MAXJ>::MAXN / 16; ++t) s.kmask[t] = 0u;
+        for (int t = 0; t < IslandShared<MAXM, MAXJ, NTH>::MAXN / 16; ++t) s.kmask[t] = 0u;
         for (int bi = 0; bi < m; ++bi)
             for (int q = s.nbr_start[bi]; q < s.nbr_start[bi + 1]; ++q) {
                 int bk = s.nbr[q];
@@ -754,7 +790,7 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
     }
     __syncthreads();
 
-    IslandSolver<MAXM, MAXJ> sv{s, a};
+    IslandSolver<MAXM, MAXJ, NTH> sv{s, a};
     sv.m = m;
     sv.J = J;
     sv.n = n;
@@ -762,8 +798,6 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
     sv.mfma = a.ns_engine == 0;
     sv.isl = isl;
     sv.first_body = b0;
-    sv.ix = 1;
-    sv.ixn = 2;
     sv.dt = a.dt;
     sv.warp = a.warp;
     sv.xoff = xoff;
@@ -798,7 +832,7 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
     }
     __syncthreads();
     double org[3] = {s.bcast[2], s.bcast[3], s.bcast[4]};
-    for (int t = threadIdx.x; t < m; t += NT) {
+    for (int t = threadIdx.x; t < m; t += NTH) {
         double p[3];
         body_position(s.pose[t], p);
         double* c = a.charges_out + (size_t)(b0 + t) * 8;
@@ -810,8 +844,8 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
         c[7] = a.size[b0 + t];
     }
     // ── write the state back ──
-    for (int t = threadIdx.x; t < m * 8; t += NT) a.pose[(size_t)(b0 + t / 8) * 8 + t % 8] = s.pose[t / 8][t % 8];
-    for (int t = threadIdx.x; t < m * 6; t += NT) a.mom[(size_t)(b0 + t / 6) * 6 + t % 6] = s.mom[t / 6][t % 6];
+    for (int t = threadIdx.x; t < m * 8; t += NTH) a.pose[(size_t)(b0 + t / 8) * 8 + t % 8] = s.pose[t / 8][t % 8];
+    for (int t = threadIdx.x; t < m * 6; t += NTH) a.mom[(size_t)(b0 + t / 6) * 6 + t % 6] = s.mom[t / 6][t % 6];
     if (threadIdx.x == 0) {
         a.seeded[isl] = s.ibcast[1];
         int* st = a.stats + (size_t)isl * ISLAND_STATS;
@@ -824,10 +858,54 @@ __global__ __launch_bounds__(NT) void island_step_kernel(IslandArgs a) {
     }
 }
 
-void launch_island_step(const IslandArgs& a, int max_m, int max_j, hipStream_t stream) {
-    (void)max_m;
-    (void)max_j;
-    hipLaunchKernelGGL((island_step_kernel<8, 16>), dim3(a.n_islands), dim3(NT), 0, stream, a);
+// Kernel variants by island class (the host routes each island to the smallest
+// that holds it; grids run over that class's island ids):
+//   m1    single bodies without joints: 64 threads, no joint code compiled in
+//   m4j8  up to 4 bodies / 8 joints: 128 threads
+//   m8j14 up to 8 bodies / 14 joints: the shared state fits two workgroups per CU
+//         (two islands in flight, 2 waves per SIMD, ≤ 256 registers per lane)
+//   m8j16 up to 8 bodies / 16 joints: one workgroup holds the CU
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m1(IslandArgs a) {
+    __shared__ IslandShared<1, 0, 64> s;
+    island_step<1, 0, 64>(a, s);
+}
+__global__ __launch_bounds__(128) void island_step_kernel_m4j8(IslandArgs a) {
+    __shared__ IslandShared<4, 8, 128> s;
+    island_step<4, 8, 128>(a, s);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14(IslandArgs a) {
+    __shared__ IslandShared<8, 14, 256> s;
+    island_step<8, 14, 256>(a, s);
+}
+__global__ __launch_bounds__(256) void island_step_kernel_m8j16(IslandArgs a) {
+    __shared__ IslandShared<8, 16, 256> s;
+    island_step<8, 16, 256>(a, s);
+}
+
+int island_class(int m, int j) {
+    if (m == 1 && j == 0) return ISLAND_CLASS_M1;
+    if (m <= 4 && j <= 8) return ISLAND_CLASS_M4J8;
+    if (m <= 8 && j <= 14) return ISLAND_CLASS_M8J14;
+    if (m <= 8 && j <= 16) return ISLAND_CLASS_M8J16;
+    return -1;
+}
+
+void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream) {
+    if (a.n_islands <= 0) return;
+    switch (cls) {
+        case ISLAND_CLASS_M1:
+            hipLaunchKernelGGL(island_step_kernel_m1, dim3(a.n_islands), dim3(64), 0, stream, a);
+            break;
+        case ISLAND_CLASS_M4J8:
+            hipLaunchKernelGGL(island_step_kernel_m4j8, dim3(a.n_islands), dim3(128), 0, stream, a);
+            break;
+        case ISLAND_CLASS_M8J14:
+            hipLaunchKernelGGL(island_step_kernel_m8j14, dim3(a.n_islands), dim3(256), 0, stream, a);
+            break;
+        default:
+            hipLaunchKernelGGL(island_step_kernel_m8j16, dim3(a.n_islands), dim3(256), 0, stream, a);
+            break;
+    }
 }
 
 }  // namespace exg

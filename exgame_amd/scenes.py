@@ -227,9 +227,11 @@ def chain_joints(first_body: int, links: np.ndarray, k: float, c: float, c_tors:
 
 
 def jointed_chains(n_chains: int = 512, m: int = 8, seed: int = 0, radius: float = 50.0, k: float = 1e3,
-                   c: float = 10.0, c_tors: float = 5.0, eps: float = 1e-6) -> Scene:
+                   c: float = 10.0, c_tors: float = 5.0, eps: float = 1e-6, braced: bool = False) -> Scene:
     """Chains of `m` rigid bodies (4,096 bodies = 512 chains of 8 by default), each
-    chain one island."""
+    chain one island. With `braced` (m ≥ 4), each chain also gets a critically damped
+    warped axial spring from body 0 to body 2 and a perpendicular damper from body 1
+    to body 3: 2(m−1) + 2 joints per island and all four force laws in play."""
     st = Stream(seed, 2)
     pos, links = chain_bodies(st, n_chains, m, radius)
     n = n_chains * m
@@ -237,8 +239,14 @@ def jointed_chains(n_chains: int = 512, m: int = 8, seed: int = 0, radius: float
     joints = []
     for ch in range(n_chains):
         joints += chain_joints(ch * m, links[ch], k, c, c_tors, eps)
+        if braced and m >= 4:
+            b0 = ch * m
+            rest = float(links[ch][0] + links[ch][1])
+            joints.append((CRITICALLY_DAMPED_WARPED, b0, b0 + 2, (0.0, 0.0, 0.0, 0.0, 0.0, 0.0, rest, 0.5 * k, c, eps)))
+            joints.append((PERPENDICULAR_DAMPER_WARPED, b0 + 1, b0 + 3, (0.5 * c, eps)))
     return _assemble(pos, rot, mass, tens, vb, np.full(n_chains, m, dtype=np.int64), joints,
-                     {"workload": "jointed_chains", "n_chains": n_chains, "m": m, "seed": seed, "radius": radius})
+                     {"workload": "jointed_chains", "n_chains": n_chains, "m": m, "seed": seed, "radius": radius,
+                      "braced": braced})
 
 
 def mixed_world(n_bodies: int, seed: int, radius: float | None = None, rigs: bool = False, center=(0.0, 0.0, 0.0),

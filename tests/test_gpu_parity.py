@@ -97,6 +97,19 @@ def test_chain_lengths_first_steps(m):
     np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
 
 
+@pytest.mark.parametrize("engine", ENGINES)
+def test_braced_chains_all_four_laws(engine):
+    # 16 joints per island (axial, torsional, critically damped warped, perpendicular):
+    # the one-island-per-CU kernel variant, every force law inside the fused solve
+    scene = scenes.jointed_chains(n_chains=8, m=8, seed=4, braced=True)
+    assert np.diff(scene.island_joint_start).max() == 16
+    dev, orc = run_pair(scene, 3, ns_engine=engine)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-11
+    np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
+
+
 def test_rigs_and_chains_first_steps():
     scene = scenes.mixed_world(160, seed=9, rigs=True)
     dev, orc = run_pair(scene, 2)
