@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libexgame_hip.so")
 
 EXG_OK = 0
 STATUS = {
-    1: "NoShader", 2: "Trap", 3: "Fatal", 4: "WorkerGone", 5: "Backend", 6: "Invalid", 7: "Unsupported",
+    1: "NoShader", 2: "Trap", 3: "Fatal", 4: "WorkerGone", 5: "Backend", 6: "Invalid", 7: "Unsupported", 8: "Full",
 }
 
 

@@ -8,6 +8,7 @@
 #include <vector>
 #include "../../include/exgame.h"
 #include "island.h"
+#include "devbuf.h"
 
 namespace {
 
@@ -25,21 +26,7 @@ int fail(int code, const std::string& msg) {
             return fail(EXG_ERR_BACKEND, std::string(#expr) + ": " + hipGetErrorString(e_));           \
     } while (0)
 
-template <class T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    hipError_t alloc(size_t count) {
-        n = count;
-        if (count == 0) return hipSuccess;
-        return hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T));
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
+using exg::DevBuf;
 
 constexpr int MAX_ISLAND_BODIES = 8;
 constexpr int MAX_ISLAND_JOINTS = 16;
@@ -47,6 +34,45 @@ constexpr int MAX_ISLAND_JOINTS = 16;
 }  // namespace
 
 int exg::set_error(int code, const char* msg) { return fail(code, msg); }
+
+int exg::check_world(const char* who, int n_bodies, int n_islands, const int* island_start, int n_joints,
+                     const int* kind, const int* a, const int* b, std::vector<int>& jstart, int& max_m, int& max_j) {
+    if (island_start[0] != 0 || island_start[n_islands] != n_bodies)
+        return fail(EXG_ERR_INVALID, std::string(who) + ": island_start must run from 0 to n_bodies");
+    // joints grouped by island, in declaration order, both ends in one island
+    std::vector<int> island_of(n_bodies);
+    max_m = 0;
+    for (int i = 0; i < n_islands; ++i) {
+        int s = island_start[i], e = island_start[i + 1];
+        if (e <= s) return fail(EXG_ERR_INVALID, std::string(who) + ": empty island " + std::to_string(i));
+        max_m = std::max(max_m, e - s);
+        for (int k = s; k < e; ++k) island_of[k] = i;
+    }
+    jstart.assign(n_islands + 1, 0);
+    int prev = 0;
+    for (int q = 0; q < n_joints; ++q) {
+        if (a[q] < 0 || a[q] >= n_bodies || b[q] < 0 || b[q] >= n_bodies || a[q] == b[q])
+            return fail(EXG_ERR_INVALID, std::string(who) + ": joint " + std::to_string(q) + " has a bad body index");
+        int ia = island_of[a[q]];
+        if (island_of[b[q]] != ia)
+            return fail(EXG_ERR_INVALID, std::string(who) + ": joint " + std::to_string(q) + " spans two islands");
+        if (ia < prev) return fail(EXG_ERR_INVALID, std::string(who) + ": joints must be grouped island by island");
+        if (kind[q] < 0 || kind[q] > 3) return fail(EXG_ERR_NO_SHADER, std::string(who) + ": unknown joint kind");
+        prev = ia;
+        jstart[ia + 1]++;
+    }
+    max_j = 0;
+    for (int i = 0; i < n_islands; ++i) {
+        max_j = std::max(max_j, jstart[i + 1]);
+        jstart[i + 1] += jstart[i];
+    }
+    if (max_m > MAX_ISLAND_BODIES || max_j > MAX_ISLAND_JOINTS)
+        return fail(EXG_ERR_UNSUPPORTED, std::string(who) + ": islands up to 8 bodies and 16 joints are supported (got " +
+                                             std::to_string(max_m) + " bodies, " + std::to_string(max_j) + " joints)");
+
+    return EXG_OK;
+}
+
 
 struct exg_world {
     int B = 0, I = 0, J = 0;
@@ -61,8 +87,8 @@ struct exg_world {
     int class_n[exg::ISLAND_CLASSES] = {};
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
-    DevBuf<float> grav_narrow;
-    exg::GravityPlan grav{};
+    DevBuf<exg::GravEntry> grav_entries;
+    DevBuf<exg::GravTile> grav_tiles;
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<long long> xacc_off;
@@ -77,7 +103,8 @@ struct exg_world {
                         &grav_partial})
             b->release();
         for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
-        grav_narrow.release();
+        grav_entries.release();
+        grav_tiles.release();
         xacc_off.release();
         for (auto& c : class_ids) c.release();
         for (auto e : ev) (void)hipEventDestroy(e);
@@ -115,38 +142,11 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
         return fail(EXG_ERR_INVALID, "exg_world_create: null state array");
     if (n_joints > 0 && (!kind || !a || !b || !params))
         return fail(EXG_ERR_INVALID, "exg_world_create: null joint array");
-    if (island_start[0] != 0 || island_start[n_islands] != n_bodies)
-        return fail(EXG_ERR_INVALID, "exg_world_create: island_start must run from 0 to n_bodies");
-    // joints grouped by island, in declaration order, both ends in one island
-    std::vector<int> island_of(n_bodies);
-    int max_m = 0;
-    for (int i = 0; i < n_islands; ++i) {
-        int s = island_start[i], e = island_start[i + 1];
-        if (e <= s) return fail(EXG_ERR_INVALID, "exg_world_create: empty island " + std::to_string(i));
-        max_m = std::max(max_m, e - s);
-        for (int k = s; k < e; ++k) island_of[k] = i;
-    }
-    std::vector<int> jstart(n_islands + 1, 0);
-    int prev = 0;
-    for (int q = 0; q < n_joints; ++q) {
-        if (a[q] < 0 || a[q] >= n_bodies || b[q] < 0 || b[q] >= n_bodies || a[q] == b[q])
-            return fail(EXG_ERR_INVALID, "exg_world_create: joint " + std::to_string(q) + " has a bad body index");
-        int ia = island_of[a[q]];
-        if (island_of[b[q]] != ia)
-            return fail(EXG_ERR_INVALID, "exg_world_create: joint " + std::to_string(q) + " spans two islands");
-        if (ia < prev) return fail(EXG_ERR_INVALID, "exg_world_create: joints must be grouped island by island");
-        if (kind[q] < 0 || kind[q] > 3) return fail(EXG_ERR_NO_SHADER, "exg_world_create: unknown joint kind");
-        prev = ia;
-        jstart[ia + 1]++;
-    }
-    int max_j = 0;
-    for (int i = 0; i < n_islands; ++i) {
-        max_j = std::max(max_j, jstart[i + 1]);
-        jstart[i + 1] += jstart[i];
-    }
-    if (max_m > MAX_ISLAND_BODIES || max_j > MAX_ISLAND_JOINTS)
-        return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: islands up to 8 bodies and 16 joints are supported (got " +
-                                             std::to_string(max_m) + " bodies, " + std::to_string(max_j) + " joints)");
+    std::vector<int> jstart;
+    int max_m = 0, max_j = 0;
+    if (int rc = exg::check_world("exg_world_create", n_bodies, n_islands, island_start, n_joints, kind, a, b, jstart,
+                                  max_m, max_j))
+        return rc;
 
     auto* w = new exg_world();
     w->B = n_bodies;
@@ -179,8 +179,14 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     {
         int cus = 0;
         W_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
-        w->grav = exg::gravity_plan(n_bodies, cus);
-        W_TRY(w->grav_partial.alloc(w->grav.partial_doubles));
+        std::vector<exg::GravEntry> entries;
+        std::vector<exg::GravTile> tiles;
+        exg::gravity_add_world(entries, tiles, 0, n_bodies, G, cus);
+        W_TRY(w->grav_entries.alloc(entries.size()));
+        W_TRY(w->grav_entries.upload(entries.data(), entries.size()));
+        W_TRY(w->grav_tiles.alloc(tiles.size()));
+        W_TRY(w->grav_tiles.upload(tiles.data(), tiles.size()));
+        W_TRY(w->grav_partial.alloc(entries.size() * exg::GRAV_PARTIAL_PER_ENTRY));
     }
     W_TRY(w->island_start.alloc(I + 1));
     W_TRY(w->island_jstart.alloc(I + 1));
@@ -270,10 +276,6 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian) {
     if (!known(gravity) || !known(jacobian)) return fail(EXG_ERR_INVALID, "exg_world_set_precision: unknown precision");
     if (jacobian != EXG_PREC_F64)
         return fail(EXG_ERR_UNSUPPORTED, "exg_world_set_precision: f32 Jacobian assembly is not available");
-    if (gravity == EXG_PREC_F32_COMPENSATED && !w->grav_narrow.p) {
-        hipError_t e = w->grav_narrow.alloc((size_t)w->B * 6);
-        if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_world_set_precision: ") + hipGetErrorString(e));
-    }
     w->grav_prec = gravity;
     w->jac_prec = jacobian;
     return EXG_OK;
@@ -304,13 +306,10 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         double* back = w->charges[1 - w->front].p;
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
         if (w->gravity) {
-            if (w->grav_prec == EXG_PREC_F32_COMPENSATED)
-                exg::launch_gravity_f32(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->grav_narrow.p, w->ext.p,
-                                        w->stream);
-            else
-                exg::launch_gravity_f64(front, w->B, 1.0 / w->G, w->grav, w->grav_partial.p, w->ext.p, w->stream);
+            exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p,
+                                (int)w->grav_tiles.n, w->grav_prec, w->grav_partial.p, w->ext.p, w->stream);
         } else {
-            HIP_TRY(hipMemsetAsync(w->ext.p, 0, (size_t)w->B * 6 * sizeof(double), w->stream));
+            exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream);
         }
         HIP_TRY(hipGetLastError());
         if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 1], w->stream));
@@ -322,7 +321,9 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         a.size = w->size.p;
         a.ext = w->ext.p;
         a.island_start = w->island_start.p;
+        a.island_end = w->island_start.p + 1;
         a.island_jstart = w->island_jstart.p;
+        a.island_jend = w->island_jstart.p + 1;
         a.origin = w->origin.p;
         a.jkind = w->jkind.p;
         a.ja = w->ja.p;
@@ -422,25 +423,28 @@ int exg_gravity_accumulate_precision(int n, const double* charges, double G, int
         return fail(EXG_ERR_INVALID, "exg_gravity_accumulate: unknown precision");
     int cus = 0;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
-    const exg::GravityPlan plan = exg::gravity_plan(n, cus);
-    double *dc = nullptr, *dw = nullptr, *dp = nullptr;
-    float* dn = nullptr;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&dc), (size_t)n * 8 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dw), (size_t)n * 6 * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&dp), plan.partial_doubles * sizeof(double));
-    if (e == hipSuccess && precision == EXG_PREC_F32_COMPENSATED)
-        e = hipMalloc(reinterpret_cast<void**>(&dn), (size_t)n * 6 * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(dc, charges, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice);
+    std::vector<exg::GravEntry> entries;
+    std::vector<exg::GravTile> tiles;
+    exg::gravity_add_world(entries, tiles, 0, n, G, cus);
+    DevBuf<double> dc, dw, dp;
+    DevBuf<exg::GravEntry> de;
+    DevBuf<exg::GravTile> dt;
+    hipError_t e = dc.alloc((size_t)n * 8);
+    if (e == hipSuccess) e = dw.alloc((size_t)n * 6);
+    if (e == hipSuccess) e = dp.alloc(entries.size() * exg::GRAV_PARTIAL_PER_ENTRY);
+    if (e == hipSuccess) e = de.alloc(entries.size());
+    if (e == hipSuccess) e = dt.alloc(tiles.size());
+    if (e == hipSuccess) e = dc.upload(charges, (size_t)n * 8);
+    if (e == hipSuccess) e = de.upload(entries.data(), entries.size());
+    if (e == hipSuccess) e = dt.upload(tiles.data(), tiles.size());
     if (e == hipSuccess) {
-        if (precision == EXG_PREC_F32_COMPENSATED) exg::launch_gravity_f32(dc, n, 1.0 / G, plan, dp, dn, dw, nullptr);
-        else exg::launch_gravity_f64(dc, n, 1.0 / G, plan, dp, dw, nullptr);
+        exg::launch_gravity(dc.p, de.p, (int)entries.size(), dt.p, (int)tiles.size(), precision, dp.p, dw.p, nullptr);
         e = hipGetLastError();
     }
-    if (e == hipSuccess) e = hipMemcpy(wrench, dw, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
-    if (dc) (void)hipFree(dc);
-    if (dw) (void)hipFree(dw);
-    if (dp) (void)hipFree(dp);
-    if (dn) (void)hipFree(dn);
+    if (e == hipSuccess) e = hipMemcpy(wrench, dw.p, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
+    for (auto* b : {&dc, &dw, &dp}) b->release();
+    de.release();
+    dt.release();
     if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_gravity_accumulate: ") + hipGetErrorString(e));
     return EXG_OK;
 }

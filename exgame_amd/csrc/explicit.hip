@@ -42,8 +42,8 @@ __device__ void explicit_body(double pose[8], double mom[6], double mass, const 
 __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs a, int integrator) {
     const int isl = blockIdx.x * EXPL_THREADS + threadIdx.x;
     if (isl >= a.n_islands) return;
-    const int b0 = a.island_start[isl], m = a.island_start[isl + 1] - b0;
-    const int j0 = a.island_jstart[isl], J = a.island_jstart[isl + 1] - j0;
+    const int b0 = a.island_start[isl], m = a.island_end[isl] - b0;
+    const int j0 = a.island_jstart[isl], J = a.island_jend[isl] - j0;
     if (m > EXPL_MAXM || J > EXPL_MAXJ || m <= 0) return;  // the host routes islands by size; never taken
     double pose[EXPL_MAXM][8], mom[EXPL_MAXM][6], wv[EXPL_MAXM][6], total[EXPL_MAXM][6];
     for (int i = 0; i < m; ++i) {

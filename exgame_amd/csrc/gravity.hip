@@ -8,17 +8,26 @@
 //
 // Layout: the charge of body i is 8 doubles [x y z | lx ly lz | G·m | size] (position
 // relative to the world frame, lever arm from the island origin, charge, core size).
+// Slots are body indices: bodies of one world are contiguous and in slot order.
 //
-// Work split: a 2-D grid of (target tile of 256, source chunk). Each thread owns one
-// target and walks its chunk's sources, staged through LDS in tiles of 256 together
-// with s² and 1/s of each source; partial sums go to partial[chunk][axis][body]
-// (coalesced) and a second kernel adds the chunks in ascending order, scales by the
-// target's charge and forms the moment. The chunk count is chosen so the grid holds
-// about four workgroups per CU, which fills all 256 CUs at a few thousand bodies.
+// Work split: a table of entries, each one (tile of ≤ 256 targets, chunk of
+// sources of the same world). A workgroup takes one entry: each thread owns a
+// target and walks the chunk's sources, staged through LDS in blocks of 256 with
+// s² and 1/s of each source, and writes its partial sum to partial[entry][axis][lane]
+// (coalesced). A second table lists, per tile, its consecutive entries: the reduce
+// kernel adds them in ascending chunk order, scales by the target's charge / G and
+// forms the moment. A world's chunking depends only on its own body count, so its
+// forces are bit-identical whatever other worlds share the launch. About four
+// workgroups per CU fill all 256 CUs from a few thousand bodies up.
 //
-// Per pair: one hardware reciprocal square root refined by two Newton steps
+// f64 pairs: one hardware reciprocal square root refined by two Newton steps
 // (y ← y + y·(1 − d²y²)/2) replaces the reference's sqrt and three divisions; the
 // result agrees with the correctly rounded form to a few ulp.
+// f32c pairs (mixed precision): sources and target are narrowed to f32 as they are
+// staged; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
+// component is accumulated as an unevaluated sum hi + lo — after every addition the
+// rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to lo.
+// The chunk sums are widened to f64 for the reduction, scaling and moment.
 #include <hip/hip_runtime.h>
 #include "island.h"
 
@@ -36,14 +45,21 @@ __device__ inline double rsqrt_refined(double x) {
     return y;
 }
 
-__global__ __launch_bounds__(GRAV_T) void gravity_partial_kernel(const double* __restrict__ ch, int n, int chunk_len,
-                                                                 double* __restrict__ partial) {
+__device__ inline void two_sum_acc(float& hi, float& lo, float x) {
+    const float t = hi + x;
+    const float bp = t - hi;
+    const float e = (hi - (t - bp)) + (x - bp);
+    hi = t;
+    lo = lo + e;
+}
+
+__global__ __launch_bounds__(GRAV_T) void gravity_partial_f64_kernel(const double* __restrict__ ch,
+                                                                     const GravEntry* __restrict__ entries,
+                                                                     double* __restrict__ partial) {
     __shared__ double sx[GRAV_T], sy[GRAV_T], sz[GRAV_T], sg[GRAV_T], ss2[GRAV_T], sinv[GRAV_T];
-    const int i = blockIdx.x * GRAV_T + threadIdx.x;
-    const int chunk = blockIdx.y;
-    const int c0 = chunk * chunk_len;
-    const int c1 = min(n, c0 + chunk_len);
-    const bool live = i < n;
+    const GravEntry en = entries[blockIdx.x];
+    const int i = en.tgt0 + threadIdx.x;
+    const bool live = (int)threadIdx.x < en.ntgt;
     double xi = 0.0, yi = 0.0, zi = 0.0, si2 = 1.0, sinv_i = 1.0;
     if (live) {
         xi = ch[(size_t)i * 8 + 0];
@@ -54,9 +70,9 @@ __global__ __launch_bounds__(GRAV_T) void gravity_partial_kernel(const double* _
         sinv_i = 1.0 / si;
     }
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int base = c0; base < c1; base += GRAV_T) {
+    for (int base = en.src0; base < en.src1; base += GRAV_T) {
         const int j = base + threadIdx.x;
-        if (j < c1) {
+        if (j < en.src1) {
             sx[threadIdx.x] = ch[(size_t)j * 8 + 0];
             sy[threadIdx.x] = ch[(size_t)j * 8 + 1];
             sz[threadIdx.x] = ch[(size_t)j * 8 + 2];
@@ -66,7 +82,7 @@ __global__ __launch_bounds__(GRAV_T) void gravity_partial_kernel(const double* _
             sinv[threadIdx.x] = 1.0 / sj;
         }
         __syncthreads();
-        const int cnt = min(GRAV_T, c1 - base);
+        const int cnt = min(GRAV_T, en.src1 - base);
         if (live) {
             for (int q = 0; q < cnt; ++q) {
                 const int jj = base + q;
@@ -87,84 +103,40 @@ __global__ __launch_bounds__(GRAV_T) void gravity_partial_kernel(const double* _
         __syncthreads();
     }
     if (live) {
-        double* p = partial + (size_t)chunk * 3 * n;
-        p[i] = fx;
-        p[(size_t)n + i] = fy;
-        p[(size_t)2 * n + i] = fz;
+        double* p = partial + (size_t)blockIdx.x * 3 * GRAV_T + threadIdx.x;
+        p[0] = fx;
+        p[GRAV_T] = fy;
+        p[2 * GRAV_T] = fz;
     }
 }
 
-__global__ __launch_bounds__(GRAV_T) void gravity_reduce_kernel(const double* __restrict__ ch, int n, int chunks,
-                                                                double g_inv, const double* __restrict__ partial,
-                                                                double* __restrict__ ext) {
-    const int i = blockIdx.x * GRAV_T + threadIdx.x;
-    if (i >= n) return;
-    double f[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < chunks; ++c)
-        for (int k = 0; k < 3; ++k) f[k] += partial[((size_t)c * 3 + k) * n + i];
-    const double a = ch[(size_t)i * 8 + 6] * g_inv;
-    for (int k = 0; k < 3; ++k) f[k] *= a;
-    double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
-    double* o = ext + (size_t)i * 6;
-    o[0] = f[0];
-    o[1] = f[1];
-    o[2] = f[2];
-    o[3] = ly * f[2] - lz * f[1];
-    o[4] = lz * f[0] - lx * f[2];
-    o[5] = lx * f[1] - ly * f[0];
-}
-
-// ───── mixed precision: FP32 pair arithmetic, two-sum compensated FP32 accumulators ─────
-// Each step the f64 charges are narrowed once to (x, y, z, G·m) and (s², 1/s) in
-// f32; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
-// component is accumulated as an unevaluated sum hi + lo: after every addition
-// the rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to
-// lo. The per-chunk sums are widened to f64 for the ordered chunk reduction, the
-// scaling by the target's charge and the moment, which stay f64.
-__device__ inline void two_sum_acc(float& hi, float& lo, float x) {
-    const float t = hi + x;
-    const float bp = t - hi;
-    const float e = (hi - (t - bp)) + (x - bp);
-    hi = t;
-    lo = lo + e;
-}
-
-__global__ __launch_bounds__(GRAV_T) void gravity_narrow_kernel(const double* __restrict__ ch, int n,
-                                                                float4* __restrict__ pos_g,
-                                                                float2* __restrict__ core) {
-    const int i = blockIdx.x * GRAV_T + threadIdx.x;
-    if (i >= n) return;
-    const double* c = ch + (size_t)i * 8;
-    pos_g[i] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-    const float sz = (float)c[7];
-    core[i] = make_float2(sz * sz, 1.0f / sz);
-}
-
-__global__ __launch_bounds__(GRAV_T) void gravity_partial_f32_kernel(const float4* __restrict__ pos_g,
-                                                                     const float2* __restrict__ core, int n,
-                                                                     int chunk_len, double* __restrict__ partial) {
+__global__ __launch_bounds__(GRAV_T) void gravity_partial_f32c_kernel(const double* __restrict__ ch,
+                                                                      const GravEntry* __restrict__ entries,
+                                                                      double* __restrict__ partial) {
     __shared__ float4 sp[GRAV_T];
     __shared__ float2 sc[GRAV_T];
-    const int i = blockIdx.x * GRAV_T + threadIdx.x;
-    const int chunk = blockIdx.y;
-    const int c0 = chunk * chunk_len;
-    const int c1 = min(n, c0 + chunk_len);
-    const bool live = i < n;
+    const GravEntry en = entries[blockIdx.x];
+    const int i = en.tgt0 + threadIdx.x;
+    const bool live = (int)threadIdx.x < en.ntgt;
     float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 mc = make_float2(1.f, 1.f);
     if (live) {
-        me = pos_g[i];
-        mc = core[i];
+        const double* c = ch + (size_t)i * 8;
+        me = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
+        const float s = (float)c[7];
+        mc = make_float2(s * s, 1.0f / s);
     }
     float fx = 0.f, fy = 0.f, fz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-    for (int base = c0; base < c1; base += GRAV_T) {
+    for (int base = en.src0; base < en.src1; base += GRAV_T) {
         const int j = base + threadIdx.x;
-        if (j < c1) {
-            sp[threadIdx.x] = pos_g[j];
-            sc[threadIdx.x] = core[j];
+        if (j < en.src1) {
+            const double* c = ch + (size_t)j * 8;
+            sp[threadIdx.x] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
+            const float s = (float)c[7];
+            sc[threadIdx.x] = make_float2(s * s, 1.0f / s);
         }
         __syncthreads();
-        const int cnt = min(GRAV_T, c1 - base);
+        const int cnt = min(GRAV_T, en.src1 - base);
         if (live) {
             for (int q = 0; q < cnt; ++q) {
                 const int jj = base + q;
@@ -184,48 +156,94 @@ __global__ __launch_bounds__(GRAV_T) void gravity_partial_f32_kernel(const float
         __syncthreads();
     }
     if (live) {
-        double* p = partial + (size_t)chunk * 3 * n;
-        p[i] = (double)fx + (double)ex;
-        p[(size_t)n + i] = (double)fy + (double)ey;
-        p[(size_t)2 * n + i] = (double)fz + (double)ez;
+        double* p = partial + (size_t)blockIdx.x * 3 * GRAV_T + threadIdx.x;
+        p[0] = (double)fx + (double)ex;
+        p[GRAV_T] = (double)fy + (double)ey;
+        p[2 * GRAV_T] = (double)fz + (double)ez;
     }
 }
 
-GravityPlan gravity_plan(int n, int compute_units) {
-    GravityPlan p{};
-    if (n <= 0) return p;
-    const int tiles = (n + GRAV_T - 1) / GRAV_T;
+__global__ __launch_bounds__(GRAV_T) void gravity_reduce_kernel(const double* __restrict__ ch,
+                                                                const GravTile* __restrict__ tiles,
+                                                                const double* __restrict__ partial,
+                                                                double* __restrict__ ext) {
+    const GravTile tl = tiles[blockIdx.x];
+    if ((int)threadIdx.x >= tl.ntgt) return;
+    const int i = tl.tgt0 + threadIdx.x;
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < tl.ne; ++c)
+        for (int k = 0; k < 3; ++k) f[k] += partial[((size_t)(tl.e0 + c) * 3 + k) * GRAV_T + threadIdx.x];
+    const double a = ch[(size_t)i * 8 + 6] * tl.g_inv;
+    for (int k = 0; k < 3; ++k) f[k] *= a;
+    double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
+    double* o = ext + (size_t)i * 6;
+    o[0] = f[0];
+    o[1] = f[1];
+    o[2] = f[2];
+    o[3] = ly * f[2] - lz * f[1];
+    o[4] = lz * f[0] - lx * f[2];
+    o[5] = lx * f[1] - ly * f[0];
+}
+
+// Bodies of worlds without gravity get a zero wrench (a fill kernel, so a step
+// can be captured in a graph).
+__global__ __launch_bounds__(GRAV_T) void zero_wrench_kernel(double* __restrict__ ext, int b0, int n) {
+    const int t = blockIdx.x * GRAV_T + threadIdx.x;
+    if (t < n * 6) ext[(size_t)b0 * 6 + t] = 0.0;
+}
+
+void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
+                       int compute_units) {
+    if (n <= 0) return;
+    const int ntiles = (n + GRAV_T - 1) / GRAV_T;
     const int want = 4 * (compute_units > 0 ? compute_units : 256);
-    int chunks = (want + tiles - 1) / tiles;
+    int chunks = (want + ntiles - 1) / ntiles;
     const int max_chunks = (n + GRAV_MIN_CHUNK - 1) / GRAV_MIN_CHUNK;
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
-    p.chunk_len = (n + chunks - 1) / chunks;
-    p.chunks = (n + p.chunk_len - 1) / p.chunk_len;
-    p.tiles = tiles;
-    p.partial_doubles = (size_t)p.chunks * 3 * n;
-    return p;
+    gravity_add_world_chunked(entries, tiles, b0, n, G, (n + chunks - 1) / chunks);
 }
 
-void launch_gravity_f32(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
-                        float* narrow, double* ext, hipStream_t stream) {
+void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
+                               int chunk_len) {
     if (n <= 0) return;
-    float4* pos_g = reinterpret_cast<float4*>(narrow);
-    float2* core = reinterpret_cast<float2*>(narrow + (size_t)4 * n);
-    hipLaunchKernelGGL(gravity_narrow_kernel, dim3(plan.tiles), dim3(GRAV_T), 0, stream, charges, n, pos_g, core);
-    hipLaunchKernelGGL(gravity_partial_f32_kernel, dim3(plan.tiles, plan.chunks), dim3(GRAV_T), 0, stream, pos_g,
-                       core, n, plan.chunk_len, partial);
-    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(plan.tiles), dim3(GRAV_T), 0, stream, charges, n, plan.chunks,
-                       g_inv, partial, ext);
+    const int ntiles = (n + GRAV_T - 1) / GRAV_T;
+    if (chunk_len < 1) chunk_len = n;
+    const int chunks = (n + chunk_len - 1) / chunk_len;
+    for (int t = 0; t < ntiles; ++t) {
+        GravTile tl{};
+        tl.tgt0 = b0 + t * GRAV_T;
+        tl.ntgt = std::min(GRAV_T, n - t * GRAV_T);
+        tl.e0 = (int)entries.size();
+        tl.ne = chunks;
+        tl.g_inv = 1.0 / G;
+        for (int c = 0; c < chunks; ++c) {
+            GravEntry e{};
+            e.tgt0 = tl.tgt0;
+            e.ntgt = tl.ntgt;
+            e.src0 = b0 + c * chunk_len;
+            e.src1 = b0 + std::min(n, (c + 1) * chunk_len);
+            entries.push_back(e);
+        }
+        tiles.push_back(tl);
+    }
 }
 
-void launch_gravity_f64(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
-                        double* ext, hipStream_t stream) {
+void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
+                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream) {
+    if (n_entries <= 0 || n_tiles <= 0) return;
+    if (precision == 1)
+        hipLaunchKernelGGL(gravity_partial_f32c_kernel, dim3(n_entries), dim3(GRAV_T), 0, stream, charges, entries,
+                           partial);
+    else
+        hipLaunchKernelGGL(gravity_partial_f64_kernel, dim3(n_entries), dim3(GRAV_T), 0, stream, charges, entries,
+                           partial);
+    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_T), 0, stream, charges, tiles, partial, ext);
+}
+
+void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(gravity_partial_kernel, dim3(plan.tiles, plan.chunks), dim3(GRAV_T), 0, stream, charges, n,
-                       plan.chunk_len, partial);
-    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(plan.tiles), dim3(GRAV_T), 0, stream, charges, n, plan.chunks,
-                       g_inv, partial, ext);
+    hipLaunchKernelGGL(zero_wrench_kernel, dim3((n * 6 + GRAV_T - 1) / GRAV_T), dim3(GRAV_T), 0, stream, ext, b0, n);
 }
 
 }  // namespace exg

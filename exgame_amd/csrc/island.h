@@ -1,11 +1,18 @@
 // Launch interface of the device kernels of the dynamics hot path.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 
 namespace exg {
 
 // Record the calling thread's last error (read back by exg_last_error) and return `code`.
 int set_error(int code, const char* msg);
+
+// Validates a world's topology (bodies per island, joints grouped by island with
+// both ends inside it, known kinds, on-chip size limits); fills the joint prefix
+// jstart[I+1] and the largest island. Returns EXG_OK or the error (message recorded).
+int check_world(const char* who, int n_bodies, int n_islands, const int* island_start, int n_joints,
+                const int* kind, const int* a, const int* b, std::vector<int>& jstart, int& max_m, int& max_j);
 
 constexpr int ISLAND_THREADS = 256;
 constexpr int ISLAND_STATS = 8;  // spans, newton iterations, NS iterations, reseeds, line-search rejects, unconverged commits
@@ -18,8 +25,10 @@ struct IslandArgs {
     const double* tens;       // B×9 body-frame inertia tensor
     const double* size;       // B effective size
     const double* ext;        // B×6 external wrench, island frame
-    const int* island_start;  // I+1
-    const int* island_jstart; // I+1
+    const int* island_start;  // first body of each island
+    const int* island_end;    // one past its last body (a world passes island_start + 1)
+    const int* island_jstart; // first joint of each island
+    const int* island_jend;   // one past its last joint
     double* origin;           // I×3
     const int* jkind;         // J
     const int* ja;            // J, global body index
@@ -34,6 +43,7 @@ struct IslandArgs {
     int ns_warm, ns_seed, n_islands;
     int ns_engine;            // 0: Newton–Schulz products on MFMA; 1: VALU in the oracle's summation order
     const int* island_ids;    // islands this launch covers (n_islands of them); null: 0 .. n_islands-1
+    const double* G_of_island;// per-island gravitational constant for the published charges; null: G
 };
 
 // Island classes of the fused Newton kernel and the launch of one class over the
@@ -45,19 +55,26 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream);
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);
 
-// Pairwise gravity: ext[i] = (Σ_j F_ij, local_i × Σ_j F_ij), charges B×8. The
-// sources are split into `chunks` chunks of `chunk_len`; partial holds
-// partial_doubles doubles of per-chunk sums.
-struct GravityPlan {
-    int tiles, chunks, chunk_len;
-    size_t partial_doubles;
+// Pairwise gravity over a work table (gravity.hip): entries pair a tile of ≤ 256
+// targets with a chunk of sources of the same world; tiles list their entries in
+// ascending chunk order. gravity_add_world appends one world's bodies [b0, b0+n)
+// with a chunking that depends only on n (and the CU count).
+struct GravEntry {
+    int tgt0, ntgt, src0, src1;
 };
-GravityPlan gravity_plan(int n, int compute_units);
-void launch_gravity_f64(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
-                        double* ext, hipStream_t stream);
-// Mixed precision: FP32 pairs with compensated accumulation; `narrow` holds 6n floats
-// (x, y, z, G·m | s², 1/s per body, 16-byte aligned).
-void launch_gravity_f32(const double* charges, int n, double g_inv, const GravityPlan& plan, double* partial,
-                        float* narrow, double* ext, hipStream_t stream);
+struct GravTile {
+    int tgt0, ntgt, e0, ne;
+    double g_inv;
+};
+constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * 256;   // doubles of partial sums per entry
+void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
+                       int compute_units);
+// The same with a fixed chunk length (≥ 64 sources).
+void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
+                               int chunk_len);
+// precision 0: f64 pairs; 1: f32 pairs with compensated accumulation
+void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
+                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream);
+void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream);
 
 }  // namespace exg

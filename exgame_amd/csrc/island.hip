@@ -117,6 +117,38 @@ __device__ double block_sum(IslandShared<MAXM, MAXJ, NTH>& s, double v) {
     return uni(r);
 }
 
+// One Jacobian column of a joint (forward-mode tangents along one axis) and, on
+// request, the wrench values. Axes 0-5 / 6-11 perturb the pose of end a / b (unit
+// left retraction), axes 12-17 / 18-23 its velocity. All pointers address LDS.
+// Inlined: an out-of-line version, storing its results to LDS through flat
+// pointers from inside the call, gave wrong solves in one kernel instantiation
+// (those stores were not reliably visible after the caller's barrier).
+__device__ __forceinline__ void joint_column(int kind, const double* params, const double* pose_a,
+                                          const double* pose_b, const double* vel_a, const double* vel_b, int ax,
+                                          double dt, double warp, double* da, double* db, double* va,
+                                          double* vb) {
+    using D = Dual<double>;
+    D pp[2][8], vv[2][6];
+    const double* poses[2] = {pose_a, pose_b};
+    const double* vels[2] = {vel_a, vel_b};
+    for (int e = 0; e < 2; ++e) {
+        if (ax < 12 && ax / 6 == e) seed_pose(poses[e], ax % 6, pp[e]);
+        else for (int k = 0; k < 8; ++k) pp[e][k] = D(poses[e][k]);
+        for (int k = 0; k < 6; ++k) vv[e][k] = D(vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
+    }
+    D wa[6], wb[6];
+    joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+    for (int k = 0; k < 6; ++k) {
+        da[k] = wa[k].d;
+        db[k] = wb[k].d;
+    }
+    if (va)
+        for (int k = 0; k < 6; ++k) {
+            va[k] = wa[k].v;
+            vb[k] = wb[k].v;
+        }
+}
+
 template <int MAXM, int MAXJ, int NTH>
 struct IslandSolver {
     IslandShared<MAXM, MAXJ, NTH>& s;
@@ -155,29 +187,11 @@ struct IslandSolver {
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
-        using D = Dual<double>;
         for (int t = threadIdx.x; t < J * 24; t += NTH) {
-            int j = s.jorder[t / 24], ax = t % 24;
-            int ends[2] = {s.ja[j], s.jb[j]};
-            D pp[2][8], vv[2][6];
-            for (int e = 0; e < 2; ++e) {
-                const double* base = s.mid0[ends[e]];
-                if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-                else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-                const double* v = s.vmid0[ends[e]];
-                for (int k = 0; k < 6; ++k) vv[e][k] = D(v[k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
-            }
-            D wa[6], wb[6];
-            joint_wrench(s.jkind[j], s.jparams[j], pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
-            for (int k = 0; k < 6; ++k) {
-                s.u.jac[j][0][ax][k] = wa[k].d;
-                s.u.jac[j][1][ax][k] = wb[k].d;
-            }
-            if (ax == 0)
-                for (int k = 0; k < 6; ++k) {
-                    s.conn0[j][0][k] = wa[k].v;
-                    s.conn0[j][1][k] = wb[k].v;
-                }
+            const int j = s.jorder[t / 24], ax = t % 24, ea = s.ja[j], eb = s.jb[j];
+            joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp,
+                         s.u.jac[j][0][ax], s.u.jac[j][1][ax], ax == 0 ? s.conn0[j][0] : nullptr,
+                         ax == 0 ? s.conn0[j][1] : nullptr);
         }
     }
     // ───── value-only joints for lanes [l0, l1) ─────
@@ -702,8 +716,8 @@ template <int MAXM, int MAXJ, int NTH>
 __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH>& s) {
     if ((int)blockIdx.x >= a.n_islands) return;
     const int isl = a.island_ids ? a.island_ids[blockIdx.x] : (int)blockIdx.x;
-    const int b0 = a.island_start[isl], m = a.island_start[isl + 1] - b0;
-    const int j0 = a.island_jstart[isl], J = a.island_jstart[isl + 1] - j0;
+    const int b0 = a.island_start[isl], m = a.island_end[isl] - b0;
+    const int j0 = a.island_jstart[isl], J = a.island_jend[isl] - j0;
     if (m > MAXM || J > MAXJ || m <= 0) return;  // the host routes islands by size; never taken
 
     // ── load the island ──
@@ -840,7 +854,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
             c[k] = p[k] + org[k];
             c[3 + k] = p[k];
         }
-        c[6] = a.G * s.massv[t];
+        c[6] = (a.G_of_island ? a.G_of_island[isl] : a.G) * s.massv[t];
         c[7] = a.size[b0 + t];
     }
     // ── write the state back ──

@@ -38,7 +38,8 @@ enum exg_status {
     EXG_ERR_WORKER_GONE = 4,  /* EvalError::WorkerGone — the device context is gone */
     EXG_ERR_BACKEND = 5,      /* EvalError::Backend — a HIP runtime call failed */
     EXG_ERR_INVALID = 6,      /* bad arguments (sizes, indices, topology) */
-    EXG_ERR_UNSUPPORTED = 7   /* a shape this build does not handle (e.g. island too large) */
+    EXG_ERR_UNSUPPORTED = 7,  /* a shape this build does not handle (e.g. island too large) */
+    EXG_ERR_FULL = 8          /* an arena has no room for the world now (retire one and retry) */
 };
 
 /* Joint kinds — the four force laws of crates/joints (MessageKind families,
@@ -203,6 +204,43 @@ enum { EXG_PREC_F64 = 0, EXG_PREC_F32_COMPENSATED = 1 };
 int exg_gravity_accumulate_precision(int n, const double* charges, double G, int precision, double* wrench);
 /* Per-world precision of gravity and of the Jacobian assembly. */
 int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
+
+/* ───────────── arenas: many independent worlds stepped together ─────────────
+ * The batched / served form of a Driver epoch stepping several mechanisms
+ * (crates/newton/src/driver.rs; accelerator/mod.rs:161-277, 613-630 flush their rows
+ * together). An arena owns device arrays of fixed capacity (bodies, islands, joints,
+ * and doubles of accepted inverses, Σ (6m)² per island); worlds are admitted into
+ * first-fit ranges of them and retired to free them. A step advances every admitted
+ * world by one implicit Newton step (gravity within each world only). A world's
+ * result is bit-identical alone or among any others. Newton integrator only. */
+typedef struct exg_arena exg_arena;
+int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity, long long inverse_capacity,
+                     exg_arena** out);
+int exg_arena_destroy(exg_arena* r);
+/* Solver of every world: tolerance², Newton–Schulz budgets and engine, gravity precision. */
+int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, int ns_engine, int gravity_precision);
+/* Gravity work split: sources per chunk (≥ 64), the same for every world. */
+int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len);
+/* Admit a world (arrays as for exg_world_create); EXG_ERR_FULL when no range fits. */
+int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double* momentum, const double* mass,
+                    const double* inertia, const double* size, int n_islands, const int* island_start,
+                    const double* origin, int n_joints, const int* kind, const int* a, const int* b,
+                    const double* params, double G, int gravity_on, int* world_id);
+int exg_arena_retire(exg_arena* r, int world_id);
+/* Advance every admitted world n_steps steps of dt (asynchronous). */
+int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp);
+int exg_arena_sync(exg_arena* r);
+/* One world's state: pose[n*8], momentum[n*6], origin[islands*3]; any may be NULL. */
+int exg_arena_download(exg_arena* r, int world_id, double* pose, double* momentum, double* origin);
+/* One world's per-island solver counters, islands*8 (as exg_world_stats). */
+int exg_arena_world_stats(exg_arena* r, int world_id, int32_t* stats);
+/* Device time of the last exg_arena_step call, milliseconds. */
+int exg_arena_last_ms(exg_arena* r, double* ms);
+/* Used and total capacity: bodies, islands, joints, inverse doubles. */
+int exg_arena_usage(exg_arena* r, int64_t* used4, int64_t* capacity4);
+/* HBM bytes a world of this shape occupies in an arena with the given gravity chunk. */
+int exg_arena_footprint(int n_bodies, int n_islands, int n_joints, long long inverse_doubles, int gravity_chunk,
+                        int64_t* bytes);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,124 @@
+"""Arenas: many independent worlds stepped together (the batched and served forms).
+
+A world's trajectory must not depend on what else is in the arena: the same world
+stepped alone and among others that come and go ends bit-identical. Each world is
+also checked against the oracle and against the single-world path."""
+import numpy as np
+import pytest
+
+import pyoracle
+from exgame_amd import scenes
+from exgame_amd.arena import Arena, inverse_doubles
+from exgame_amd.world import DeviceWorld
+
+from test_gpu_parity import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def big_arena(**kw):
+    return Arena(bodies=1 << 15, islands=1 << 13, joints=1 << 14, inverses=1 << 22, **kw)
+
+
+def test_arena_world_matches_oracle():
+    scene = scenes.jointed_chains(n_chains=4, m=8, seed=0)
+    ar = big_arena(tol2=1e-20)
+    wid = ar.admit(scene)
+    ar.step(2, 1e-2, 1.0)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    orc.step(2, 1e-2, 1.0)
+    a, b = ar.state(wid), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-12
+    np.testing.assert_array_equal(ar.stats(wid)[:, :6], orc.stats())
+
+
+def test_arena_equals_single_world_path_bitwise():
+    # 32 bodies: both paths put the sources in one chunk, so gravity is the same sum
+    scene = scenes.jointed_chains(n_chains=4, m=8, seed=2)
+    ar = big_arena(tol2=1e-20)
+    wid = ar.admit(scene)
+    dev = DeviceWorld(scene, tol2=1e-20)
+    ar.step(5, 1e-2, 1.0)
+    dev.step(5, 1e-2, 1.0)
+    a, b = ar.state(wid), dev.state()
+    assert np.array_equal(a["pose"], b["pose"]) and np.array_equal(a["momentum"], b["momentum"])
+    assert np.array_equal(a["origin"], b["origin"])
+
+
+def test_alone_and_in_a_changing_mix_are_bit_identical():
+    target = scenes.mixed_world(300, seed=11, rigs=True)
+    others = [scenes.jointed_chains(n_chains=6, m=8, seed=20), scenes.mixed_world(700, seed=21),
+              scenes.plain_gravity(n=150, seed=22)]
+    alone = big_arena(tol2=1e-20)
+    wa = alone.admit(target)
+    alone.step(10, 1e-2, 1.0)
+
+    mix = big_arena(tol2=1e-20)
+    w0 = mix.admit(others[0])
+    wt = mix.admit(target)
+    mix.step(3, 1e-2, 1.0)
+    w1 = mix.admit(others[1])
+    mix.step(3, 1e-2, 1.0)
+    mix.retire(w0)
+    w2 = mix.admit(others[2], G=0.5)
+    mix.step(4, 1e-2, 1.0)
+    a, b = alone.state(wa), mix.state(wt)
+    for k in ("pose", "momentum", "origin"):
+        assert np.array_equal(a[k], b[k]), k
+    np.testing.assert_array_equal(alone.stats(wa), mix.stats(wt))
+    assert w1 is not None and w2 is not None
+
+
+def test_admission_refuses_when_full_and_reuses_freed_ranges():
+    s8 = scenes.jointed_chains(n_chains=2, m=8, seed=5)   # 16 bodies, 2 islands
+    ar = Arena(bodies=40, islands=6, joints=64, inverses=8 * 48 * 48, tol2=1e-12)
+    a = ar.admit(s8)
+    b = ar.admit(s8)
+    assert a is not None and b is not None
+    assert ar.admit(s8) is None                  # 48 bodies > 40
+    used, cap = ar.usage()
+    assert used[0] == 32 and cap[0] == 40
+    ar.retire(a)
+    c = ar.admit(s8)                             # first fit takes the freed range
+    assert c is not None
+    used, _ = ar.usage()
+    assert used[0] == 32
+    ar.step(2, 1e-2, 1.0)
+    assert np.all(np.isfinite(ar.state(c)["pose"]))
+    with pytest.raises(Exception):
+        ar.retire(a)                             # already retired
+
+
+def test_worlds_without_gravity_and_with_their_own_G():
+    scene = scenes.plain_gravity(n=64, seed=3)
+    ar = big_arena(tol2=1e-20)
+    w_off = ar.admit(scene, gravity=False)
+    w_g2 = ar.admit(scene, G=2.0)
+    ar.step(3, 1e-3, 1.0)
+    orc_off = pyoracle.OracleWorld(scene, tol2=1e-20, gravity=False)
+    orc_g2 = pyoracle.OracleWorld(scene, tol2=1e-20, G=2.0)
+    orc_off.step(3, 1e-3, 1.0)
+    orc_g2.step(3, 1e-3, 1.0)
+    assert rel(ar.state(w_off)["position"], orc_off.state()["position"]) < 1e-12
+    assert rel(ar.state(w_g2)["position"], orc_g2.state()["position"]) < 1e-12
+    assert inverse_doubles(scene) == 64 * 36
+
+
+def test_served_requests_equal_their_alone_runs():
+    from exgame_amd.arena import footprint
+    from exgame_amd.serve import Server, arena_for, synthetic_requests
+    reqs = synthetic_requests(10, rate=400.0, seed=3, bodies=(64, 600), steps=(15, 45), median_bodies=200.0)
+    budget = int(0.4 * sum(footprint(r.scene) for r in reqs))   # some requests must wait for room
+    ar = arena_for(reqs, budget, tol2=1e-20)
+    rep = Server(ar, budget, tick_steps=7, keep_results=True).run(reqs)
+    assert rep["requests"] == len(reqs)
+    assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
+    assert any(r.admitted > r.arrival + 1e-4 for r in reqs)      # queued behind the budget
+    for r in reqs[:4]:
+        alone = big_arena(tol2=1e-20)
+        w = alone.admit(r.scene)
+        alone.step(r.steps, 1e-2, 1.0)
+        a = alone.state(w)
+        for k in ("pose", "momentum", "origin"):
+            assert np.array_equal(a[k], r.result[k]), (r.rid, k)
