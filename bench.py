@@ -75,6 +75,17 @@ def cpu_baseline(target_s: float = 10.0):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def measured_traffic():
+    """HBM bytes per island-kernel launch from the committed counter passes (copy-
+    calibrated FETCH_SIZE and WRITE_SIZE), or None when no such record exists."""
+    path = os.path.join(HERE, "profiles", "island_step_traffic_jointed_chains_4096.json")
+    try:
+        with open(path) as f:
+            return int(json.load(f)["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def copy_calibration(nbytes: int) -> None:
     """One device-to-device copy of `nbytes` through the HIP runtime (its blit kernel
     reads and writes exactly nbytes), so HBM byte counters can be scaled."""
@@ -167,7 +178,7 @@ def main() -> int:
                             "ns_iterations_per_island": float(ns_it.mean() / args.steps),
                             "spans_per_island": float(stats[:, 0].mean() / args.steps)},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "island_step_kernel (Newton-Schulz + matvec flops / avg launch)"},
     }
     if not args.no_cpu_baseline:

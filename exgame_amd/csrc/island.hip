@@ -223,10 +223,13 @@ struct IslandSolver {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = block[r][c];
     }
-    __device__ void gather_post(int l0, int l1) {
+    // Candidate 0's world mass block goes to mass0 only when mass0 may take it: in
+    // the second line-search round candidate 0 is lane 4, and mass0 must keep lane
+    // 0's block unless a later lane is accepted (the oracle's lanes[0].mass).
+    __device__ void gather_post(int l0, int l1, bool mass_of_candidate0 = true) {
         for (int t = threadIdx.x; t < (l1 - l0) * m; t += NTH) {
             int lane = l0 + t / m, i = t % m;
-            gather_post_one(lane, i, lane == 0);
+            gather_post_one(lane, i, lane == 0 && mass_of_candidate0);
         }
     }
     static __device__ double max2(double x, double y) { return x > y ? x : y; }
@@ -564,7 +567,7 @@ struct IslandSolver {
                 __syncthreads();
                 joints_plain(0, LS);
                 __syncthreads();
-                gather_post(0, LS);
+                gather_post(0, LS, round == 0);
                 __syncthreads();
                 wnorm(0, LS);
                 __syncthreads();
@@ -577,13 +580,15 @@ struct IslandSolver {
                     }
             }
             if (accepted >= 0) {
-                if (accepted_slot > 0) {  // adopt_lane (cache.rs:1115-1136)
+                if (accepted > 0) {  // adopt_lane (cache.rs:1115-1136)
                     const int as = accepted_slot;
-                    for (int t = threadIdx.x; t < m * 6; t += NTH) {
-                        s.vmid0[t / 6][t % 6] = VMID(as, t / 6)[t % 6];
-                        s.rhs0[t / 6][t % 6] = RHS(as, t / 6)[t % 6];
+                    if (as > 0) {
+                        for (int t = threadIdx.x; t < m * 6; t += NTH) {
+                            s.vmid0[t / 6][t % 6] = VMID(as, t / 6)[t % 6];
+                            s.rhs0[t / 6][t % 6] = RHS(as, t / 6)[t % 6];
+                        }
+                        for (int t = threadIdx.x; t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
                     }
-                    for (int t = threadIdx.x; t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
                     // the accepted lane's mass block, recomputed bit-identically from its inputs
                     for (int t = threadIdx.x; t < m; t += NTH) {
                         double block[6][6], rr[6], sc[2];

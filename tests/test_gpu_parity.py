@@ -110,6 +110,24 @@ def test_braced_chains_all_four_laws(engine):
     np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
 
 
+@pytest.mark.parametrize("engine", ENGINES)
+def test_stiff_chains_subdivide_reject_and_reseed(engine):
+    # k = 1e5 at dt = 5e-2: hundreds of dt subdivisions, line searches whose first
+    # four lanes all fail (the second round of lanes), Newton–Schulz reseeds. The
+    # oracle-order engine must take exactly the oracle's decisions; the MFMA engine
+    # rounds its products differently, so only the trajectory is compared.
+    scene = scenes.jointed_chains(n_chains=8, m=8, seed=1, k=1e5, c=1.0)
+    dev, orc = run_pair(scene, 3, dt=5e-2, ns_engine=engine)
+    st = orc.stats()
+    assert st[:, 0].sum() > 100 and st[:, 4].sum() > 20 and st[:, 3].sum() > 20
+    a, b = dev.state(), orc.state()
+    if engine == "valu_ordered":
+        np.testing.assert_array_equal(dev.stats()[:, :6], st)
+        assert rel(a["position"], b["position"]) < 1e-12
+    else:
+        assert rel(a["position"], b["position"]) < 1e-9
+
+
 def test_rigs_and_chains_first_steps():
     scene = scenes.mixed_world(160, seed=9, rigs=True)
     dev, orc = run_pair(scene, 2)
