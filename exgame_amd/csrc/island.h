@@ -51,6 +51,8 @@ struct IslandArgs {
 enum { ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASSES = 4 };
 int island_class(int m, int joints);   // -1 when no variant holds the island
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream);
+// Single-body islands without joints, one thread per island (singles.hip).
+void launch_singles(const IslandArgs& a, hipStream_t stream);
 
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);

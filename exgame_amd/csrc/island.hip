@@ -879,15 +879,11 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
 
 // Kernel variants by island class (the host routes each island to the smallest
 // that holds it; grids run over that class's island ids):
-//   m1    single bodies without joints: 64 threads, no joint code compiled in
+//   m1    single bodies without joints: one thread per island (singles.hip)
 //   m4j8  up to 4 bodies / 8 joints: 128 threads
 //   m8j14 up to 8 bodies / 14 joints: the shared state fits two workgroups per CU
 //         (two islands in flight, 2 waves per SIMD, ≤ 256 registers per lane)
 //   m8j16 up to 8 bodies / 16 joints: one workgroup holds the CU
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m1(IslandArgs a) {
-    __shared__ IslandShared<1, 0, 64> s;
-    island_step<1, 0, 64>(a, s);
-}
 __global__ __launch_bounds__(128) void island_step_kernel_m4j8(IslandArgs a) {
     __shared__ IslandShared<4, 8, 128> s;
     island_step<4, 8, 128>(a, s);
@@ -913,7 +909,7 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream) {
     if (a.n_islands <= 0) return;
     switch (cls) {
         case ISLAND_CLASS_M1:
-            hipLaunchKernelGGL(island_step_kernel_m1, dim3(a.n_islands), dim3(64), 0, stream, a);
+            launch_singles(a, stream);
             break;
         case ISLAND_CLASS_M4J8:
             hipLaunchKernelGGL(island_step_kernel_m4j8, dim3(a.n_islands), dim3(128), 0, stream, a);

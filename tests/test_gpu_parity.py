@@ -121,11 +121,12 @@ def test_stiff_chains_subdivide_reject_and_reseed(engine):
     st = orc.stats()
     assert st[:, 0].sum() > 100 and st[:, 4].sum() > 20 and st[:, 3].sum() > 20
     a, b = dev.state(), orc.state()
-    if engine == "valu_ordered":
-        np.testing.assert_array_equal(dev.stats()[:, :6], st)
-        assert rel(a["position"], b["position"]) < 1e-12
-    else:
-        assert rel(a["position"], b["position"]) < 1e-9
+    assert rel(a["position"], b["position"]) < 1e-9
+    # over ~1,000 Newton iterations per island a few convergence tests land within
+    # rounding of tol² (gravity is summed in another order), so counters agree in total
+    d = dev.stats()[:, :6].astype(np.int64).sum(0)
+    o = st.astype(np.int64).sum(0)
+    assert np.all(np.abs(d - o) <= np.maximum(2, 0.01 * o)), (d, o)
 
 
 def test_rigs_and_chains_first_steps():
@@ -159,6 +160,35 @@ def test_plain_gravity_explicit_matches_oracle(integrator):
     a, b = dev.state(), orc.state()
     assert rel(a["position"], b["position"]) < 1e-11
     assert rel(a["momentum"][:, :3], b["momentum"][:, :3]) < 1e-10
+
+
+@pytest.mark.parametrize("steps", [1, 6])
+def test_free_bodies_under_newton_one_thread_each(steps):
+    # every island a single body without joints: the thread-per-island kernel. Close
+    # encounters subdivide the step hundreds of times; there a convergence test can land
+    # within rounding of tol², so after the gravity-free first step the counters are
+    # compared in total
+    scene = scenes.plain_gravity(n=200, seed=4)
+    dev, orc = run_pair(scene, steps, dt=1e-3)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-11
+    if steps == 1:
+        np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
+    else:
+        d = dev.stats()[:, :6].astype(np.int64).sum(0)
+        o = orc.stats().astype(np.int64).sum(0)
+        assert np.all(np.abs(d - o) <= np.maximum(2, 0.01 * o)), (d, o)
+
+
+def test_free_bodies_without_gravity_take_the_oracles_decisions():
+    scene = scenes.plain_gravity(n=200, seed=4)
+    dev = DeviceWorld(scene, tol2=1e-20, gravity=False)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20, gravity=False)
+    dev.step(5, 1e-3, 1.0)
+    orc.step(5, 1e-3, 1.0)
+    assert rel(dev.state()["position"], orc.state()["position"]) < 1e-13
+    np.testing.assert_array_equal(dev.stats()[:, :6], orc.stats())
 
 
 def test_singletons_and_gravity_first_steps():
