@@ -181,6 +181,25 @@ def main() -> int:
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": measured_traffic(),
                      "kernel": "island_step_kernel (Newton-Schulz + matvec flops / avg launch)"},
     }
+    # the same rollout in mixed precision, against the f64 final state
+    ref = world.state()["position"]
+    mixed = {}
+    for name, gp, jp in [("f32_jacobian", "f64", "f32c"), ("f32_jacobian_f32c_gravity", "f32c", "f32c")]:
+        w = DeviceWorld(scene, tol2=TOL * TOL, gravity_precision=gp, jacobian_precision=jp)
+        w.step(min(args.warmup, 1), H, 1.0)   # code objects of this variant
+        w.close()
+        w = DeviceWorld(scene, tol2=TOL * TOL, gravity_precision=gp, jacobian_precision=jp)
+        w.sync()
+        t1 = time.perf_counter()
+        w.step(args.steps, H, 1.0)
+        w.sync()
+        e1 = time.perf_counter() - t1
+        pos = w.state()["position"]
+        err = float(np.linalg.norm(pos - ref) / np.linalg.norm(ref))
+        mixed[name] = {"value": body_steps / e1, "speedup_vs_f64": el / e1, "final_state_rel_error": err,
+                       "meets_1e-9": bool(err <= 1e-9), "gravity": gp, "jacobian": jp}
+        w.close()
+    line["mixed_precision"] = mixed
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()

@@ -21,6 +21,7 @@ _SIGS = {
     "exg_arena_destroy": [C.c_void_p],
     "exg_arena_set_solver": [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "exg_arena_set_gravity_chunk": [C.c_void_p, C.c_int],
+    "exg_arena_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "exg_arena_admit": [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp, C.c_int, _ip, _ip, _ip, _dp,
                         C.c_double, C.c_int, _ip],
     "exg_arena_retire": [C.c_void_p, C.c_int],
@@ -60,7 +61,7 @@ def footprint(scene, gravity_chunk: int = 1024) -> int:
 class Arena:
     def __init__(self, bodies: int, islands: int, joints: int, inverses: int, tol2: float = 1e-20,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
-                 gravity_chunk: int = 1024):
+                 gravity_chunk: int = 1024, jacobian_precision: str = "f64"):
         self.L = _bind()
         h = C.c_void_p()
         _lib.check(self.L.exg_arena_create(int(bodies), int(islands), int(joints), int(inverses), C.byref(h)))
@@ -69,6 +70,8 @@ class Arena:
         _lib.check(self.L.exg_arena_set_solver(self.h, float(tol2), int(ns_warm), int(ns_seed), NS_ENGINES[ns_engine],
                                                PRECISIONS[gravity_precision]))
         _lib.check(self.L.exg_arena_set_gravity_chunk(self.h, self.gravity_chunk))
+        _lib.check(self.L.exg_arena_set_precision(self.h, PRECISIONS[gravity_precision],
+                                                  PRECISIONS[jacobian_precision]))
         self.shapes = {}
 
     def admit(self, scene, G: float = 1.0, gravity: bool = True):

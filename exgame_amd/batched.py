@@ -33,7 +33,8 @@ def main() -> int:
     ap.add_argument("--worlds", type=int, default=512)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32c"])
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32c"], help="gravity arithmetic")
+    ap.add_argument("--jacobian", default=None, choices=["f64", "f32c"], help="Jacobian arithmetic (default: as gravity)")
     ap.add_argument("--chunk", type=int, default=1024)
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
@@ -41,7 +42,7 @@ def main() -> int:
     B = sum(w.n_bodies for w in worlds)
     ar = Arena(bodies=B, islands=sum(w.n_islands for w in worlds), joints=sum(w.n_joints for w in worlds) + 1,
                inverses=sum(inverse_doubles(w) for w in worlds), tol2=1e-20, gravity_precision=args.precision,
-               gravity_chunk=args.chunk)
+               gravity_chunk=args.chunk, jacobian_precision=args.jacobian or args.precision)
     ids = [ar.admit(w) for w in worlds]
     assert all(i is not None for i in ids)
     ar.step(args.warmup, 1e-2, 1.0)
@@ -53,6 +54,7 @@ def main() -> int:
     line = {"form": "batched", "worlds": len(worlds), "bodies": B, "steps": args.steps,
             "body_steps_per_s": B * args.steps / el, "ms_per_step": el * 1e3 / args.steps,
             "device_ms_per_step": ar.last_ms() / args.steps, "precision": args.precision,
+            "jacobian": args.jacobian or args.precision,
             "gravity_chunk": args.chunk,
             "islands": sum(w.n_islands for w in worlds), "pairs_per_step": int(sum(w.n_bodies ** 2 for w in worlds))}
     print(json.dumps(line), flush=True)

@@ -93,7 +93,7 @@ struct exg_arena {
     int cus = 256;
     int grav_chunk = 1024;
     double tol2 = 1e-8;
-    int ns_warm = 2, ns_seed = 24, ns_engine = EXG_NS_MFMA, grav_prec = EXG_PREC_F64;
+    int ns_warm = 2, ns_seed = 24, ns_engine = EXG_NS_MFMA, grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
@@ -261,6 +261,15 @@ int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, in
     return EXG_OK;
 }
 
+int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_precision: null arena");
+    auto known = [](int p) { return p == EXG_PREC_F64 || p == EXG_PREC_F32_COMPENSATED; };
+    if (!known(gravity) || !known(jacobian)) return afail(EXG_ERR_INVALID, "exg_arena_set_precision: unknown precision");
+    r->grav_prec = gravity;
+    r->jac_prec = jacobian;
+    return EXG_OK;
+}
+
 int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_gravity_chunk: null arena");
     if (chunk_len < 64) return afail(EXG_ERR_INVALID, "exg_arena_set_gravity_chunk: chunk below 64 sources");
@@ -422,6 +431,7 @@ int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp) {
         a.ns_warm = r->ns_warm;
         a.ns_seed = r->ns_seed;
         a.ns_engine = r->ns_engine;
+        a.jac_f32 = r->jac_prec == EXG_PREC_F32_COMPENSATED;
         for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
             if (!r->class_n[c]) continue;
             a.island_ids = r->class_ids[c].p;

@@ -95,8 +95,15 @@ struct exg_world {
     int front = 0;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
-    bool timing = true;
-    double last_ms[2] = {0.0, 0.0};
+    bool timing_split = false;   // record the gravity/island split event inside a step
+    bool use_graph = false;
+    bool graph_ms_valid = false;
+    int settings_epoch = 0;      // bumped when a setting baked into kernel arguments changes
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        double dt = 0.0, warp = 0.0;
+        int settings = -1;
+    } graphs[2];                 // by the charge buffer the two-step graph starts from
 
     void release() {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
@@ -109,6 +116,8 @@ struct exg_world {
         for (auto& c : class_ids) c.release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -265,6 +274,7 @@ int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm,
     if (ns_warm < 0 || ns_seed < 1) return fail(EXG_ERR_INVALID, "exg_world_set_solver: bad Newton–Schulz budgets");
     w->integrator = integrator;
     w->tol2 = tol2;
+    w->settings_epoch++;
     w->ns_warm = ns_warm;
     w->ns_seed = ns_seed;
     return EXG_OK;
@@ -274,9 +284,8 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_precision: null world");
     auto known = [](int p) { return p == EXG_PREC_F64 || p == EXG_PREC_F32_COMPENSATED; };
     if (!known(gravity) || !known(jacobian)) return fail(EXG_ERR_INVALID, "exg_world_set_precision: unknown precision");
-    if (jacobian != EXG_PREC_F64)
-        return fail(EXG_ERR_UNSUPPORTED, "exg_world_set_precision: f32 Jacobian assembly is not available");
     w->grav_prec = gravity;
+    w->settings_epoch++;
     w->jac_prec = jacobian;
     return EXG_OK;
 }
@@ -286,78 +295,148 @@ int exg_world_set_ns_engine(exg_world* w, int engine) {
     if (engine != EXG_NS_MFMA && engine != EXG_NS_VALU_ORDERED)
         return fail(EXG_ERR_INVALID, "exg_world_set_ns_engine: unknown engine " + std::to_string(engine));
     w->ns_engine = engine;
+    w->settings_epoch++;
+    return EXG_OK;
+}
+
+// Enqueue one world step on the world's stream (no event records: also used inside
+// graph capture).
+static int enqueue_world_step(exg_world* w, double dt, double warp) {
+    double* front = w->charges[w->front].p;
+    double* back = w->charges[1 - w->front].p;
+    if (w->gravity) {
+        exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p, (int)w->grav_tiles.n,
+                            w->grav_prec, w->grav_partial.p, w->ext.p, w->stream);
+    } else {
+        exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    if (w->timing_split) HIP_TRY(hipEventRecord(w->ev[3 * w->ev_used + 1], w->stream));
+    exg::IslandArgs a{};
+    a.pose = w->pose.p;
+    a.mom = w->mom.p;
+    a.mass = w->mass.p;
+    a.tens = w->tens.p;
+    a.size = w->size.p;
+    a.ext = w->ext.p;
+    a.island_start = w->island_start.p;
+    a.island_end = w->island_start.p + 1;
+    a.island_jstart = w->island_jstart.p;
+    a.island_jend = w->island_jstart.p + 1;
+    a.origin = w->origin.p;
+    a.jkind = w->jkind.p;
+    a.ja = w->ja.p;
+    a.jb = w->jb.p;
+    a.jparams = w->jparams.p;
+    a.xacc = w->xacc.p;
+    a.xacc_off = w->xacc_off.p;
+    a.seeded = w->seeded.p;
+    a.charges_out = back;
+    a.stats = w->stats.p;
+    a.dt = dt;
+    a.warp = warp;
+    a.tol2 = w->tol2;
+    a.G = w->G;
+    a.ns_warm = w->ns_warm;
+    a.ns_seed = w->ns_seed;
+    a.ns_engine = w->ns_engine;
+    a.jac_f32 = w->jac_prec == EXG_PREC_F32_COMPENSATED;
+    a.n_islands = w->I;
+    if (w->integrator == EXG_NEWTON) {
+        for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
+            if (!w->class_n[c]) continue;
+            a.island_ids = w->class_ids[c].p;
+            a.n_islands = w->class_n[c];
+            exg::launch_island_class(a, c, w->stream);
+        }
+    } else {
+        a.island_ids = nullptr;
+        exg::launch_explicit_step(a, w->integrator, w->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    w->front = 1 - w->front;
+    return EXG_OK;
+}
+
+// Graph of two consecutive steps starting from charge buffer `front` (one round trip
+// of the double buffer, so the graph replays without re-binding pointers).
+static int world_graph(exg_world* w, double dt, double warp, hipGraphExec_t* out) {
+    auto& g = w->graphs[w->front];
+    if (g.exec && g.dt == dt && g.warp == warp && g.settings == w->settings_epoch) {
+        *out = g.exec;
+        return EXG_OK;
+    }
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    const int front0 = w->front;
+    const bool split = w->timing_split;
+    w->timing_split = false;
+    HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_world_step(w, dt, warp);
+    if (rc == EXG_OK) rc = enqueue_world_step(w, dt, warp);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(w->stream, &graph);
+    w->timing_split = split;
+    w->front = front0;
+    if (rc != EXG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    g.dt = dt;
+    g.warp = warp;
+    g.settings = w->settings_epoch;
+    *out = g.exec;
     return EXG_OK;
 }
 
 int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_step: null world");
     if (n_steps < 0) return fail(EXG_ERR_INVALID, "exg_world_step: negative step count");
-    if (w->timing) {
-        size_t need = (size_t)n_steps * 3;
-        while (w->ev.size() < need) {
+    w->ev_used = 0;
+    w->graph_ms_valid = false;
+    if (w->use_graph && n_steps >= 2) {
+        // whole-call timing only: the graph replays two steps per launch
+        while (w->ev.size() < 2) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             w->ev.push_back(e);
         }
+        hipGraphExec_t exec = nullptr;
+        if (int rc = world_graph(w, dt, warp, &exec)) return rc;
+        HIP_TRY(hipEventRecord(w->ev[0], w->stream));
+        for (int k = 0; k + 1 < n_steps; k += 2) HIP_TRY(hipGraphLaunch(exec, w->stream));
+        if (n_steps % 2) {
+            w->timing_split = false;
+            if (int rc = enqueue_world_step(w, dt, warp)) return rc;
+        }
+        HIP_TRY(hipEventRecord(w->ev[1], w->stream));
+        w->graph_ms_valid = true;
+        return EXG_OK;
     }
-    w->ev_used = 0;
+    size_t need = (size_t)n_steps * 3;
+    while (w->ev.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        w->ev.push_back(e);
+    }
+    w->timing_split = true;
     for (int s = 0; s < n_steps; ++s) {
-        double* front = w->charges[w->front].p;
-        double* back = w->charges[1 - w->front].p;
-        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
-        if (w->gravity) {
-            exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p,
-                                (int)w->grav_tiles.n, w->grav_prec, w->grav_partial.p, w->ext.p, w->stream);
-        } else {
-            exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream);
-        }
-        HIP_TRY(hipGetLastError());
-        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 1], w->stream));
-        exg::IslandArgs a{};
-        a.pose = w->pose.p;
-        a.mom = w->mom.p;
-        a.mass = w->mass.p;
-        a.tens = w->tens.p;
-        a.size = w->size.p;
-        a.ext = w->ext.p;
-        a.island_start = w->island_start.p;
-        a.island_end = w->island_start.p + 1;
-        a.island_jstart = w->island_jstart.p;
-        a.island_jend = w->island_jstart.p + 1;
-        a.origin = w->origin.p;
-        a.jkind = w->jkind.p;
-        a.ja = w->ja.p;
-        a.jb = w->jb.p;
-        a.jparams = w->jparams.p;
-        a.xacc = w->xacc.p;
-        a.xacc_off = w->xacc_off.p;
-        a.seeded = w->seeded.p;
-        a.charges_out = back;
-        a.stats = w->stats.p;
-        a.dt = dt;
-        a.warp = warp;
-        a.tol2 = w->tol2;
-        a.G = w->G;
-        a.ns_warm = w->ns_warm;
-        a.ns_seed = w->ns_seed;
-        a.ns_engine = w->ns_engine;
-        a.n_islands = w->I;
-        if (w->integrator == EXG_NEWTON) {
-            for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
-                if (!w->class_n[c]) continue;
-                a.island_ids = w->class_ids[c].p;
-                a.n_islands = w->class_n[c];
-                exg::launch_island_class(a, c, w->stream);
-            }
-        } else {
-            a.island_ids = nullptr;
-            exg::launch_explicit_step(a, w->integrator, w->stream);
-        }
-        HIP_TRY(hipGetLastError());
-        if (w->timing) HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
-        w->front = 1 - w->front;
+        HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
+        if (int rc = enqueue_world_step(w, dt, warp)) return rc;
+        HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
+        w->ev_used = s + 1;
     }
-    w->ev_used = w->timing ? n_steps : 0;
+    w->timing_split = false;
+    return EXG_OK;
+}
+
+int exg_world_set_graph(exg_world* w, int enable) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_graph: null world");
+    w->use_graph = enable != 0;
     return EXG_OK;
 }
 
@@ -370,6 +449,13 @@ int exg_world_sync(exg_world* w) {
 int exg_world_last_timing(exg_world* w, double* out2) {
     if (!w || !out2) return fail(EXG_ERR_INVALID, "exg_world_last_timing: null argument");
     HIP_TRY(hipStreamSynchronize(w->stream));
+    if (w->graph_ms_valid) {  // graph replay: one total, not split by kernel
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, w->ev[0], w->ev[1]));
+        out2[0] = -1.0;
+        out2[1] = t;
+        return EXG_OK;
+    }
     double g = 0.0, s = 0.0;
     for (int k = 0; k < w->ev_used; ++k) {
         float a = 0.f, b = 0.f;

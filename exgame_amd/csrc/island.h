@@ -44,6 +44,7 @@ struct IslandArgs {
     int ns_engine;            // 0: Newton–Schulz products on MFMA; 1: VALU in the oracle's summation order
     const int* island_ids;    // islands this launch covers (n_islands of them); null: 0 .. n_islands-1
     const double* G_of_island;// per-island gravitational constant for the published charges; null: G
+    int jac_f32;              // 1: FP32 Jacobian lanes and compensated FP32 assembly (mixed precision)
 };
 
 // Island classes of the fused Newton kernel and the launch of one class over the

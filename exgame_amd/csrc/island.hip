@@ -19,6 +19,7 @@
 // so the workgroup stays converged.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "island.h"
 #include "physics_dev.h"
 
@@ -68,17 +69,18 @@ struct LaneSlots {
 // (evaluation → assembly), T = 2I − A·X (inside one Newton–Schulz iteration) and
 // line-search candidates 1..3 (line search → adoption). With MAXJ = 14 an 8-body
 // island fits in 80 KB, so two islands share a CU.
-template <int MAXM, int MAXJ, int NTH>
+template <int MAXM, int MAXJ, int NTH, bool JF>
 struct IslandShared {
     static constexpr int MAXN = ((6 * MAXM + 15) / 16) * 16;
     static constexpr int LDM = MAXN + 1;
     static constexpr int NB = MAXN * LDM;
     static constexpr int JD = MAXJ > 0 ? MAXJ : 1;   // array extent (islands without joints: MAXJ = 0)
+    using JT = typename std::conditional<JF, float, double>::type;   // Jacobian storage
     double A[NB];
     double X[NB];
     union {
         double T[NB];
-        double jac[JD][2][24][6];           // ∂wrench[end]/∂axis, per joint
+        JT jac[JD][2][24][6];               // ∂wrench[end]/∂axis, per joint
         LaneSlots<MAXM, MAXJ> ls;
     } u;
     unsigned kmask[MAXN / 16];              // per 16-row tile of A: 4-column slabs that can be non-zero
@@ -104,8 +106,8 @@ struct IslandShared {
 // then the four wave sums added in wave order by every thread (the consumers compare
 // the result against a ×4 margin or take ⌊log₂⌋, never test it for equality). The
 // leading barrier protects s.red from the previous call's readers.
-template <int MAXM, int MAXJ, int NTH>
-__device__ double block_sum(IslandShared<MAXM, MAXJ, NTH>& s, double v) {
+template <int MAXM, int MAXJ, int NTH, bool JF>
+__device__ double block_sum(IslandShared<MAXM, MAXJ, NTH, JF>& s, double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     __syncthreads();
@@ -123,18 +125,21 @@ __device__ double block_sum(IslandShared<MAXM, MAXJ, NTH>& s, double v) {
 // Inlined: an out-of-line version, storing its results to LDS through flat
 // pointers from inside the call, gave wrong solves in one kernel instantiation
 // (those stores were not reliably visible after the caller's barrier).
+// T = float is the mixed-precision Jacobian: inputs narrowed to f32, f32 tangents.
+template <class T>
 __device__ __forceinline__ void joint_column(int kind, const double* params, const double* pose_a,
                                           const double* pose_b, const double* vel_a, const double* vel_b, int ax,
-                                          double dt, double warp, double* da, double* db, double* va,
-                                          double* vb) {
-    using D = Dual<double>;
+                                          double dt, double warp, T* da, T* db, double* va, double* vb) {
+    using D = Dual<T>;
     D pp[2][8], vv[2][6];
     const double* poses[2] = {pose_a, pose_b};
     const double* vels[2] = {vel_a, vel_b};
     for (int e = 0; e < 2; ++e) {
-        if (ax < 12 && ax / 6 == e) seed_pose(poses[e], ax % 6, pp[e]);
-        else for (int k = 0; k < 8; ++k) pp[e][k] = D(poses[e][k]);
-        for (int k = 0; k < 6; ++k) vv[e][k] = D(vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
+        T base[8];
+        for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
+        if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
+        else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+        for (int k = 0; k < 6; ++k) vv[e][k] = D((T)vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? T(1) : T(0));
     }
     D wa[6], wb[6];
     joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
@@ -149,11 +154,11 @@ __device__ __forceinline__ void joint_column(int kind, const double* params, con
         }
 }
 
-template <int MAXM, int MAXJ, int NTH>
+template <int MAXM, int MAXJ, int NTH, bool JF>
 struct IslandSolver {
-    IslandShared<MAXM, MAXJ, NTH>& s;
+    IslandShared<MAXM, MAXJ, NTH, JF>& s;
     const IslandArgs& a;
-    static constexpr int LD = IslandShared<MAXM, MAXJ, NTH>::LDM;
+    static constexpr int LD = IslandShared<MAXM, MAXJ, NTH, JF>::LDM;
     int m, J, isl, first_body;
     int n, n16;               // system size 6m and its 16-padded size
     bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
@@ -189,9 +194,12 @@ struct IslandSolver {
         if constexpr (MAXJ == 0) return;
         for (int t = threadIdx.x; t < J * 24; t += NTH) {
             const int j = s.jorder[t / 24], ax = t % 24, ea = s.ja[j], eb = s.jb[j];
+            // the f64 lanes also deliver the wrench values; with f32 lanes the residual's
+            // wrenches come from an f64 value-only evaluation instead
+            const bool values = !JF && ax == 0;
             joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp,
-                         s.u.jac[j][0][ax], s.u.jac[j][1][ax], ax == 0 ? s.conn0[j][0] : nullptr,
-                         ax == 0 ? s.conn0[j][1] : nullptr);
+                         s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
+                         values ? s.conn0[j][1] : nullptr);
         }
     }
     // ───── value-only joints for lanes [l0, l1) ─────
@@ -272,14 +280,33 @@ struct IslandSolver {
             double acc[6];
 #pragma unroll
             for (int c = 0; c < 6; ++c) acc[c] = 0.0 * dst[c] + dg * s.mass0[i][r * 6 + c];
-            for (int q = q0; q < q1; ++q) {
-                const int code = s.terms[q];
-                const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
+            if constexpr (JF) {
+                // FP32 terms, summed as an unevaluated hi + lo (two-sum), added in f64
+                const float pf32 = (float)pf, vf32 = (float)vf;
+                float hi[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, lo[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                for (int q = q0; q < q1; ++q) {
+                    const int code = s.terms[q];
+                    const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
 #pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    double p = s.u.jac[jn][re][ce * 6 + c][r];
-                    double v = s.u.jac[jn][re][12 + ce * 6 + c][r];
-                    acc[c] = acc[c] + pf * p + vf * v;
+                    for (int c = 0; c < 6; ++c) {
+                        const float x = pf32 * s.u.jac[jn][re][ce * 6 + c][r] + vf32 * s.u.jac[jn][re][12 + ce * 6 + c][r];
+                        const float t = hi[c] + x, bp = t - hi[c];
+                        lo[c] = lo[c] + ((hi[c] - (t - bp)) + (x - bp));
+                        hi[c] = t;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] = acc[c] + ((double)hi[c] + (double)lo[c]);
+            } else {
+                for (int q = q0; q < q1; ++q) {
+                    const int code = s.terms[q];
+                    const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) {
+                        double p = s.u.jac[jn][re][ce * 6 + c][r];
+                        double v = s.u.jac[jn][re][12 + ce * 6 + c][r];
+                        acc[c] = acc[c] + pf * p + vf * v;
+                    }
                 }
             }
 #pragma unroll
@@ -290,7 +317,7 @@ struct IslandSolver {
     // With in_place, every cell is computed before any is stored (out may be bm).
     __device__ void gemm(const double* am, const double* bm, double* out, bool two_minus, bool sparse,
                          bool in_place) {
-        constexpr int MAXCELLS = (IslandShared<MAXM, MAXJ, NTH>::MAXN * IslandShared<MAXM, MAXJ, NTH>::MAXN + NTH - 1) / NTH;
+        constexpr int MAXCELLS = (IslandShared<MAXM, MAXJ, NTH, JF>::MAXN * IslandShared<MAXM, MAXJ, NTH, JF>::MAXN + NTH - 1) / NTH;
         double cell[MAXCELLS];
         double sign = two_minus ? -1.0 : 1.0;
         int u = 0;
@@ -331,8 +358,8 @@ struct IslandSolver {
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
         const unsigned dense = (1u << KS) - 1u;
-        constexpr int MAXKS = IslandShared<MAXM, MAXJ, NTH>::MAXN / 4;
-        constexpr int MAXT = IslandShared<MAXM, MAXJ, NTH>::MAXN / 16;
+        constexpr int MAXKS = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 4;
+        constexpr int MAXT = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 16;
         constexpr int PER_WAVE = (MAXT * MAXT + NTH / 64 - 1) / (NTH / 64);
         double part = 0.0;
         // this wave's tiles: wave, wave + 4, … (wave-uniform), accumulated side by side
@@ -522,6 +549,7 @@ struct IslandSolver {
                 pre(0, 1);
                 __syncthreads();
                 joints_jacobian();
+                if constexpr (JF) joints_plain(0, 1);
                 __syncthreads();
                 gather_post(0, 1);
                 __syncthreads();
@@ -717,8 +745,8 @@ struct IslandSolver {
     }
 };
 
-template <int MAXM, int MAXJ, int NTH>
-__device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH>& s) {
+template <int MAXM, int MAXJ, int NTH, bool JF>
+__device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH, JF>& s) {
     if ((int)blockIdx.x >= a.n_islands) return;
     const int isl = a.island_ids ? a.island_ids[blockIdx.x] : (int)blockIdx.x;
     const int b0 = a.island_start[isl], m = a.island_end[isl] - b0;
@@ -741,9 +769,9 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         for (int p = 0; p < 10; ++p) s.jparams[t][p] = a.jparams[(size_t)(j0 + t) * 10 + p];
     }
     const long long xoff = a.xacc_off[isl];
-    constexpr int LD = IslandShared<MAXM, MAXJ, NTH>::LDM;
+    constexpr int LD = IslandShared<MAXM, MAXJ, NTH, JF>::LDM;
     const int n = 6 * m;
-    for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ, NTH>::NB; e += NTH) {
+    for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ, NTH, JF>::NB; e += NTH) {
         s.A[e] = 0.0;
         s.X[e] = 0.0;
         s.u.T[e] = 0.0;
@@ -799,7 +827,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
             for (int j = 0; j < J; ++j)
                 if (s.jkind[j] == k) s.jorder[o++] = j;
         // A's non-zero 4-column slabs per 16-row tile, from the block neighbour lists
-        for (int t = 0; t < IslandShared<MAXM, MAXJ, NTH>::MAXN / 16; ++t) s.kmask[t] = 0u;
+        for (int t = 0; t < IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 16; ++t) s.kmask[t] = 0u;
         for (int bi = 0; bi < m; ++bi)
             for (int q = s.nbr_start[bi]; q < s.nbr_start[bi + 1]; ++q) {
                 int bk = s.nbr[q];
@@ -809,7 +837,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     }
     __syncthreads();
 
-    IslandSolver<MAXM, MAXJ, NTH> sv{s, a};
+    IslandSolver<MAXM, MAXJ, NTH, JF> sv{s, a};
     sv.m = m;
     sv.J = J;
     sv.n = n;
@@ -885,16 +913,30 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
 //         (two islands in flight, 2 waves per SIMD, ≤ 256 registers per lane)
 //   m8j16 up to 8 bodies / 16 joints: one workgroup holds the CU
 __global__ __launch_bounds__(128) void island_step_kernel_m4j8(IslandArgs a) {
-    __shared__ IslandShared<4, 8, 128> s;
-    island_step<4, 8, 128>(a, s);
+    __shared__ IslandShared<4, 8, 128, false> s;
+    island_step<4, 8, 128, false>(a, s);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14(IslandArgs a) {
-    __shared__ IslandShared<8, 14, 256> s;
-    island_step<8, 14, 256>(a, s);
+    __shared__ IslandShared<8, 14, 256, false> s;
+    island_step<8, 14, 256, false>(a, s);
 }
 __global__ __launch_bounds__(256) void island_step_kernel_m8j16(IslandArgs a) {
-    __shared__ IslandShared<8, 16, 256> s;
-    island_step<8, 16, 256>(a, s);
+    __shared__ IslandShared<8, 16, 256, false> s;
+    island_step<8, 16, 256, false>(a, s);
+}
+// Mixed precision: FP32 Jacobian lanes and compensated FP32 assembly (f64 residual,
+// Newton–Schulz and state update).
+__global__ __launch_bounds__(128) void island_step_kernel_m4j8_jf32(IslandArgs a) {
+    __shared__ IslandShared<4, 8, 128, true> s;
+    island_step<4, 8, 128, true>(a, s);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14_jf32(IslandArgs a) {
+    __shared__ IslandShared<8, 14, 256, true> s;
+    island_step<8, 14, 256, true>(a, s);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j16_jf32(IslandArgs a) {
+    __shared__ IslandShared<8, 16, 256, true> s;
+    island_step<8, 16, 256, true>(a, s);
 }
 
 int island_class(int m, int j) {
@@ -912,13 +954,16 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream) {
             launch_singles(a, stream);
             break;
         case ISLAND_CLASS_M4J8:
-            hipLaunchKernelGGL(island_step_kernel_m4j8, dim3(a.n_islands), dim3(128), 0, stream, a);
+            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m4j8_jf32, dim3(a.n_islands), dim3(128), 0, stream, a);
+            else hipLaunchKernelGGL(island_step_kernel_m4j8, dim3(a.n_islands), dim3(128), 0, stream, a);
             break;
         case ISLAND_CLASS_M8J14:
-            hipLaunchKernelGGL(island_step_kernel_m8j14, dim3(a.n_islands), dim3(256), 0, stream, a);
+            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m8j14_jf32, dim3(a.n_islands), dim3(256), 0, stream, a);
+            else hipLaunchKernelGGL(island_step_kernel_m8j14, dim3(a.n_islands), dim3(256), 0, stream, a);
             break;
         default:
-            hipLaunchKernelGGL(island_step_kernel_m8j16, dim3(a.n_islands), dim3(256), 0, stream, a);
+            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m8j16_jf32, dim3(a.n_islands), dim3(256), 0, stream, a);
+            else hipLaunchKernelGGL(island_step_kernel_m8j16, dim3(a.n_islands), dim3(256), 0, stream, a);
             break;
     }
 }

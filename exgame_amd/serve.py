@@ -165,7 +165,7 @@ def main() -> int:
     reqs = synthetic_requests(args.requests, args.rate, args.seed, (args.min_bodies, args.max_bodies),
                               (args.min_steps, args.max_steps), args.median_bodies)
     budget = int(args.hbm_gib * (1 << 30))
-    ar = arena_for(reqs, budget, tol2=1e-20, gravity_precision=args.precision)
+    ar = arena_for(reqs, budget, tol2=1e-20, gravity_precision=args.precision, jacobian_precision=args.precision)
     rep = Server(ar, budget, tick_steps=args.tick).run(reqs)
     rep.update({"rate_per_s": args.rate, "precision": args.precision, "hbm_budget_gib": args.hbm_gib,
                 "bodies_range": [args.min_bodies, args.max_bodies], "steps_range": [args.min_steps, args.max_steps]})

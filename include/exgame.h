@@ -111,6 +111,11 @@ int exg_world_set_ns_engine(exg_world* w, int engine);
  * re-anchoring and the new charges. Asynchronous on the world's stream. */
 int exg_world_step(exg_world* w, int n_steps, double dt, double warp);
 
+/* Replay steps from a captured HIP graph of two consecutive steps (re-captured when
+ * dt, warp or a solver setting changes); exg_world_last_timing then reports only the
+ * total time (out[0] = -1). No counterpart in the reference. */
+int exg_world_set_graph(exg_world* w, int enable);
+
 /* Block until the world's queued work has finished. */
 int exg_world_sync(exg_world* w);
 
@@ -202,7 +207,10 @@ int exg_gravity_accumulate(int n, const double* charges, double G, double* wrenc
  * state update stay f64 either way. */
 enum { EXG_PREC_F64 = 0, EXG_PREC_F32_COMPENSATED = 1 };
 int exg_gravity_accumulate_precision(int n, const double* charges, double G, int precision, double* wrench);
-/* Per-world precision of gravity and of the Jacobian assembly. */
+/* Per-world precision of gravity and of the Jacobian assembly: with
+ * EXG_PREC_F32_COMPENSATED the joint Jacobians are evaluated in FP32 forward mode and
+ * assembled in FP32 with two-sum accumulation; the residual's wrenches, Newton–Schulz
+ * and the state update stay f64, so a converged step meets the same f64 tolerance. */
 int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
 
 /* ───────────── arenas: many independent worlds stepped together ─────────────
@@ -219,6 +227,8 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
 int exg_arena_destroy(exg_arena* r);
 /* Solver of every world: tolerance², Newton–Schulz budgets and engine, gravity precision. */
 int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, int ns_engine, int gravity_precision);
+/* Precision of gravity and of the Jacobian assembly for every world (EXG_PREC_*). */
+int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian);
 /* Gravity work split: sources per chunk (≥ 64), the same for every world. */
 int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len);
 /* Admit a world (arrays as for exg_world_create); EXG_ERR_FULL when no range fits. */

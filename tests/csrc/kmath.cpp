@@ -54,6 +54,30 @@ void km_joint_jacobian(int kind, const double* params, const double* poses16, co
     }
 }
 
+// The mixed-precision Jacobian: FP32 forward-mode lanes on the same inputs narrowed
+// to f32 (the island kernel's f32 Jacobian path), widened for comparison.
+void km_joint_jacobian_f32(int kind, const double* params, const double* poses16, const double* vels12, double dt,
+                           double warp, double* jac288) {
+    using D = Dual<float>;
+    for (int ax = 0; ax < 24; ++ax) {
+        D pp[2][8], vv[2][6];
+        for (int e = 0; e < 2; ++e) {
+            float base[8];
+            for (int k = 0; k < 8; ++k) base[k] = (float)poses16[8 * e + k];
+            if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
+            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+            for (int k = 0; k < 6; ++k)
+                vv[e][k] = D((float)vels12[6 * e + k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0f : 0.0f);
+        }
+        D wa[6], wb[6];
+        joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+        for (int k = 0; k < 6; ++k) {
+            jac288[(0 * 24 + ax) * 6 + k] = wa[k].d;
+            jac288[(1 * 24 + ax) * 6 + k] = wb[k].d;
+        }
+    }
+}
+
 void km_body_post(const double* pose8, const double* vel6, double mass, const double* t9, const double* snap6,
                   const double* total6, double half, double floor2, double* block36, double* rhs6, double* scale2) {
     double block[6][6];

@@ -66,3 +66,38 @@ def test_world_with_f32c_gravity_tracks_f64():
     err = rel(sb["position"], sa["position"])
     print(f"100-step final-state position error, f32c gravity vs f64: {err:.2e}")
     assert err < 5e-8
+
+
+@pytest.mark.parametrize("braced", [False, True])
+def test_f32_jacobian_converges_to_the_f64_solution(braced):
+    # FP32 Jacobian lanes and compensated FP32 assembly change only the Newton path;
+    # the residual is f64 and each step still converges to tol² = 1e-20
+    scene = scenes.jointed_chains(n_chains=12, m=8, seed=6, braced=braced)
+    dev = DeviceWorld(scene, tol2=1e-20, jacobian_precision="f32c")
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    dev.step(20, 1e-2, 1.0)
+    orc.step(20, 1e-2, 1.0)
+    err = rel(dev.state()["position"], orc.state()["position"])
+    merr = rel(dev.state()["momentum"], orc.state()["momentum"])
+    print(f"braced={braced}: 20-step error with f32 Jacobians: position {err:.2e} momentum {merr:.2e}; "
+          f"Newton iterations {int(dev.stats()[:, 1].sum())} vs {int(orc.stats()[:, 1].sum())}")
+    assert err < 1e-9 and merr < 1e-8
+    assert np.all(dev.stats()[:, 5] == 0)   # every span converged
+
+
+def test_precision_form_65536_jointed_bodies():
+    # the jointed half of the precision form: 8,192 chains of 8; f32 Jacobians alone,
+    # then with f32c gravity as well, against the all-f64 run on the device
+    scene = scenes.jointed_chains(n_chains=8192, m=8, seed=0, radius=200.0)
+    runs = {}
+    for name, gp, jp in [("f64", "f64", "f64"), ("jac32", "f64", "f32c"), ("mixed", "f32c", "f32c")]:
+        w = DeviceWorld(scene, tol2=1e-20, gravity_precision=gp, jacobian_precision=jp)
+        w.step(20, 1e-2, 1.0)
+        runs[name] = w.state()
+        w.close()
+    e_jac = rel(runs["jac32"]["position"], runs["f64"]["position"])
+    e_mix = rel(runs["mixed"]["position"], runs["f64"]["position"])
+    print(f"65,536 bodies, 20 steps: final-state position error vs f64 — f32 Jacobian {e_jac:.2e}, "
+          f"f32 Jacobian + f32c gravity {e_mix:.2e}")
+    assert e_jac < 1e-9
+    assert e_mix < 1e-6

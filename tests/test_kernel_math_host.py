@@ -115,6 +115,23 @@ def test_joints_and_jacobian_lanes(km, kind):
         assert rel(jac.reshape(2, 24, 6), wj) < 1e-12
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_f32_jacobian_lanes_track_the_f64_jacobian(km, kind):
+    # the mixed-precision Jacobian: Dual<float> lanes on f32-narrowed poses and
+    # velocities; relative error a few f32 ulps times the conditioning of the law
+    prm = PARAMS[kind]
+    worst = 0.0
+    for _ in range(5):
+        poses = np.concatenate([rmotor(), rmotor()])
+        vels = RNG.normal(size=12)
+        dt, warp = 0.011, 1.3
+        jac, jp = p(np.zeros(288))
+        km.km_joint_jacobian_f32(kind, p(prm)[1], p(poses)[1], p(vels)[1], C.c_double(dt), C.c_double(warp), jp)
+        _, wj = O.joint_jacobian(kind, prm, poses, vels, dt, warp)
+        worst = max(worst, rel(jac.reshape(2, 24, 6), wj))
+    assert worst < 2e-5, worst
+
+
 def test_body_post_matches(km):
     for _ in range(5):
         m = rmotor()
