@@ -98,6 +98,7 @@ struct exg_arena {
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
     DevBuf<long long> xacc_off;
+    DevBuf<exg::IslandTopo> topo;
     DevBuf<exg::GravEntry> entries;
     DevBuf<exg::GravTile> tiles;
     size_t n_entries = 0, n_tiles = 0;
@@ -117,6 +118,7 @@ struct exg_arena {
         for (auto* b : {&island_start, &island_end, &island_jstart, &island_jend, &jkind, &ja, &jb, &seeded, &stats})
             b->release();
         xacc_off.release();
+        topo.release();
         entries.release();
         tiles.release();
         for (auto& c : class_ids) c.release();
@@ -224,6 +226,7 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
     R_TRY(r->seeded.alloc(I));
     R_TRY(r->stats.alloc(I * exg::ISLAND_STATS));
     R_TRY(r->xacc_off.alloc(I));
+    R_TRY(r->topo.alloc(I));
     R_TRY(r->xacc.alloc((size_t)inverse_capacity));
     R_TRY(r->jkind.alloc(J));
     R_TRY(r->ja.alloc(J));
@@ -294,6 +297,13 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     if (int rc = exg::check_world("exg_arena_admit", n_bodies, n_islands, island_start, n_joints, kind, a, b, jstart,
                                   max_m, max_j))
         return rc;
+    std::vector<exg::IslandTopo> topo(n_islands);
+    for (int i = 0; i < n_islands; ++i) {
+        const int j0 = jstart[i], ib0 = island_start[i];
+        if (exg::island_topology_of(island_start[i + 1] - ib0, jstart[i + 1] - j0, kind + j0, a + j0, b + j0, ib0,
+                                    topo[i]) != 0)
+            return afail(EXG_ERR_UNSUPPORTED, "exg_arena_admit: an island exceeds the kernels' topology record");
+    }
     long long nx = 0;
     std::vector<long long> xo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
@@ -340,6 +350,7 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     A_TRY(up(r->island_jstart.p + w.i0, js.data(), I * sizeof(int)));
     A_TRY(up(r->island_jend.p + w.i0, je.data(), I * sizeof(int)));
     A_TRY(up(r->xacc_off.p + w.i0, xoff.data(), I * sizeof(long long)));
+    A_TRY(up(r->topo.p + w.i0, topo.data(), I * sizeof(exg::IslandTopo)));
     A_TRY(up(r->island_G.p + w.i0, gi.data(), I * sizeof(double)));
     A_TRY(up(r->seeded.p + w.i0, zero_i.data(), I * sizeof(int)));
     A_TRY(up(r->stats.p + w.i0 * exg::ISLAND_STATS, zero_i.data(), I * exg::ISLAND_STATS * sizeof(int)));
@@ -420,6 +431,7 @@ int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp) {
         a.jparams = r->jparams.p;
         a.xacc = r->xacc.p;
         a.xacc_off = r->xacc_off.p;
+        a.topo = r->topo.p;
         a.seeded = r->seeded.p;
         a.charges_out = back;
         a.stats = r->stats.p;
@@ -496,7 +508,8 @@ int exg_arena_footprint(int n_bodies, int n_islands, int n_joints, long long inv
     if (!bytes || n_bodies < 0 || n_islands < 0 || n_joints < 0 || inverse_doubles < 0 || gravity_chunk < 64)
         return afail(EXG_ERR_INVALID, "exg_arena_footprint: bad arguments");
     const long long per_body = (8 + 6 + 1 + 9 + 1 + 6 + 16) * 8;                 // state, ext, two charges
-    const long long per_island = 3 * 8 + 5 * 4 + exg::ISLAND_STATS * 4 + 8;       // origin, ranges, stats, offset
+    const long long per_island = 3 * 8 + 5 * 4 + exg::ISLAND_STATS * 4 + 8 +      // origin, ranges, stats, offset,
+                                 (long long)sizeof(exg::IslandTopo);             // topology record
     const long long per_joint = 10 * 8 + 3 * 4;                                   // params, kind, ends
     const long long tiles = (n_bodies + 255) / 256;
     const long long chunks = (n_bodies + gravity_chunk - 1) / gravity_chunk;

@@ -92,6 +92,7 @@ struct exg_world {
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<long long> xacc_off;
+    DevBuf<exg::IslandTopo> topo;   // per island, Newton integrator only
     int front = 0;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
@@ -113,6 +114,7 @@ struct exg_world {
         grav_entries.release();
         grav_tiles.release();
         xacc_off.release();
+        topo.release();
         for (auto& c : class_ids) c.release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -156,6 +158,13 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     if (int rc = exg::check_world("exg_world_create", n_bodies, n_islands, island_start, n_joints, kind, a, b, jstart,
                                   max_m, max_j))
         return rc;
+    std::vector<exg::IslandTopo> topo(n_islands);
+    for (int i = 0; i < n_islands; ++i) {
+        const int j0 = jstart[i], b0 = island_start[i];
+        if (exg::island_topology_of(island_start[i + 1] - b0, jstart[i + 1] - j0, kind + j0, a + j0, b + j0, b0,
+                                    topo[i]) != 0)
+            return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: an island exceeds the kernels' topology record");
+    }
 
     auto* w = new exg_world();
     w->B = n_bodies;
@@ -239,6 +248,8 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(up(w->island_start.p, island_start, (I + 1) * sizeof(int)));
     W_TRY(up(w->island_jstart.p, jstart.data(), (I + 1) * sizeof(int)));
     W_TRY(up(w->xacc_off.p, xoff.data(), I * sizeof(long long)));
+    W_TRY(w->topo.alloc(I));
+    W_TRY(w->topo.upload(topo.data(), I));
     if (Jn) {
         W_TRY(up(w->jkind.p, kind, Jn * sizeof(int)));
         W_TRY(up(w->ja.p, a, Jn * sizeof(int)));
@@ -328,6 +339,7 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.ja = w->ja.p;
     a.jb = w->jb.p;
     a.jparams = w->jparams.p;
+    a.topo = w->topo.p;
     a.xacc = w->xacc.p;
     a.xacc_off = w->xacc_off.p;
     a.seeded = w->seeded.p;

@@ -1,6 +1,7 @@
 // Launch interface of the device kernels of the dynamics hot path.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstdint>
 #include <vector>
 
 namespace exg {
@@ -13,6 +14,27 @@ int set_error(int code, const char* msg);
 // jstart[I+1] and the largest island. Returns EXG_OK or the error (message recorded).
 int check_world(const char* who, int n_bodies, int n_islands, const int* island_start, int n_joints,
                 const int* kind, const int* a, const int* b, std::vector<int>& jstart, int& max_m, int& max_j);
+
+// Tables of one island that depend only on its joint graph (neighbour lists, assembly
+// terms, incidence, joints by kind, A's non-zero slab masks and the MFMA tile lists
+// of each wave), built on the host by build_island_topology (topology.cpp) when a
+// world is set up, for islands of up to TOPO_MAXM bodies and TOPO_MAXJ joints.
+constexpr int TOPO_MAXM = 8, TOPO_MAXJ = 16, TOPO_WAVES = 4, TOPO_PER_WAVE = 4, TOPO_MAXT = 3;
+struct IslandTopo {
+    uint16_t kmask[4];                              // per 16-row tile: 4-column slabs that can be non-zero
+    uint8_t tcount[2][TOPO_WAVES];                  // [0] sparse product A·X, [1] dense product X·T
+    uint8_t tlist[2][TOPO_WAVES][TOPO_PER_WAVE];    // row tile | column tile << 4
+    uint8_t nbr_start[TOPO_MAXM + 1], nbr[TOPO_MAXM * TOPO_MAXM];
+    uint8_t term_start[TOPO_MAXM * TOPO_MAXM + 1], terms[4 * TOPO_MAXJ];   // joint << 2 | row end << 1 | column end
+    uint8_t inc_start[TOPO_MAXM + 1], inc[2 * TOPO_MAXJ];                 // joint << 1 | end
+    uint8_t jorder[TOPO_MAXJ];
+    uint8_t pad[13];
+};
+static_assert(sizeof(IslandTopo) == 320, "IslandTopo is a 320-byte record");
+// m bodies, J joints with island-local ends ja/jb and kinds; `waves` and `per_wave`
+// from island_class_shape. Returns 0, or -1 when the island exceeds the record.
+int build_island_topology(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
+                          IslandTopo& t);
 
 constexpr int ISLAND_THREADS = 256;
 constexpr int ISLAND_STATS = 8;  // spans, newton iterations, NS iterations, reseeds, line-search rejects, unconverged commits
@@ -45,12 +67,18 @@ struct IslandArgs {
     const int* island_ids;    // islands this launch covers (n_islands of them); null: 0 .. n_islands-1
     const double* G_of_island;// per-island gravitational constant for the published charges; null: G
     int jac_f32;              // 1: FP32 Jacobian lanes and compensated FP32 assembly (mixed precision)
+    const IslandTopo* topo;   // I records (islands of the single-body class need none)
 };
 
 // Island classes of the fused Newton kernel and the launch of one class over the
 // island ids in a.island_ids[0 .. a.n_islands).
 enum { ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASSES = 4 };
 int island_class(int m, int joints);   // -1 when no variant holds the island
+// Waves per workgroup and MFMA tiles per wave of a class's kernel (for its tile lists).
+void island_class_shape(int cls, int& waves, int& per_wave);
+// build_island_topology for island i of a world: bodies [b0, b0 + m), joints with
+// global body indices a/b; routes by island_class.
+int island_topology_of(int m, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t);
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream);
 // Single-body islands without joints, one thread per island (singles.hip).
 void launch_singles(const IslandArgs& a, hipStream_t stream);

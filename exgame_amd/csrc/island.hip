@@ -84,6 +84,10 @@ struct IslandShared {
         LaneSlots<MAXM, MAXJ> ls;
     } u;
     unsigned kmask[MAXN / 16];              // per 16-row tile of A: 4-column slabs that can be non-zero
+    static constexpr int MAXT = MAXN / 16, WAVES = NTH / 64;
+    static constexpr int PER_WAVE = (MAXT * MAXT + WAVES - 1) / WAVES;
+    int tlist[2][WAVES][PER_WAVE];          // MFMA tiles of each wave: [0] sparse product, [1] dense
+    int tcount[2][WAVES];
     double vmid0[MAXM][6], mid0[MAXM][8], total0[MAXM][6], rhs0[MAXM][6], scale0[MAXM][2], ratio0[MAXM][2];
     double conn0[JD][2][6];
     double wn[LS];
@@ -131,18 +135,26 @@ __device__ __forceinline__ void joint_column(int kind, const double* params, con
                                           const double* pose_b, const double* vel_a, const double* vel_b, int ax,
                                           double dt, double warp, T* da, T* db, double* va, double* vb) {
     using D = Dual<T>;
-    D pp[2][8], vv[2][6];
+    D vv[2][6], wa[6], wb[6];
     const double* poses[2] = {pose_a, pose_b};
     const double* vels[2] = {vel_a, vel_b};
-    for (int e = 0; e < 2; ++e) {
-        T base[8];
-        for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
-        if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-        else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+    for (int e = 0; e < 2; ++e)
         for (int k = 0; k < 6; ++k) vv[e][k] = D((T)vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? T(1) : T(0));
+    if (ax < 12) {  // pose axis: tangents through the geometry
+        D pp[2][8];
+        for (int e = 0; e < 2; ++e) {
+            T base[8];
+            for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
+            if (ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
+            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+        }
+        joint_wrench2<D, D>(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+    } else {  // velocity axis: the geometry carries no tangent
+        T pz[2][8];
+        for (int e = 0; e < 2; ++e)
+            for (int k = 0; k < 8; ++k) pz[e][k] = (T)poses[e][k];
+        joint_wrench2<T, D>(kind, params, pz[0], pz[1], vv[0], vv[1], dt, warp, wa, wb);
     }
-    D wa[6], wb[6];
-    joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
     for (int k = 0; k < 6; ++k) {
         da[k] = wa[k].d;
         db[k] = wb[k].d;
@@ -192,8 +204,12 @@ struct IslandSolver {
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
+        // pose columns (tangents through the geometry) first, then the cheaper velocity
+        // columns: a thread that takes a second column takes a cheap one
         for (int t = threadIdx.x; t < J * 24; t += NTH) {
-            const int j = s.jorder[t / 24], ax = t % 24, ea = s.ja[j], eb = s.jb[j];
+            const int pose_item = t < J * 12;
+            const int q = pose_item ? t : t - J * 12;
+            const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
             // the f64 lanes also deliver the wrench values; with f32 lanes the residual's
             // wrenches come from an f64 value-only evaluation instead
             const bool values = !JF && ax == 0;
@@ -352,8 +368,10 @@ struct IslandSolver {
     // Lane layout of v_mfma_f64_16x16x4f64 (measured on gfx950): lane l supplies
     // a[l%16][l/16] and b[l/16][l%16]; accumulator j holds c[l/16 + 4j][l%16].
     // With in_place, all tiles are multiplied before any is stored (out may be am).
+    // `which` selects the wave's tile list: 0 for the sparse product (tiles balanced by
+    // their count of non-zero slabs), 1 for the dense one.
     __device__ double mfma_product(const double* am, const double* bm, double* out, const unsigned* kmask,
-                                   bool make_t, bool in_place) {
+                                   bool make_t, bool in_place, int which) {
         const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
@@ -362,17 +380,22 @@ struct IslandSolver {
         constexpr int MAXT = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 16;
         constexpr int PER_WAVE = (MAXT * MAXT + NTH / 64 - 1) / (NTH / 64);
         double part = 0.0;
-        // this wave's tiles: wave, wave + 4, … (wave-uniform), accumulated side by side
+        // this wave's tiles (wave-uniform), accumulated side by side
         int ti[PER_WAVE], tj[PER_WAVE];
         unsigned mk[PER_WAVE];
         d4 acc[PER_WAVE];
+        const int ntiles = uni(s.tcount[which][wave]);
+        // every slot is read (in bounds; unused ones hold 0) and selected on scalars, so
+        // that tile indices and masks stay in scalar registers and each MFMA is guarded
+        // by a scalar branch
 #pragma unroll
         for (int u = 0; u < PER_WAVE; ++u) {
-            const int t = wave + u * (NTH / 64);
-            const bool live = t < T * T;
-            ti[u] = live ? t / T : 0;
-            tj[u] = live ? t % T : 0;
-            mk[u] = live ? (kmask ? uni(kmask[ti[u]]) : dense) : 0u;
+            const bool live = u < ntiles;
+            const int t = uni(s.tlist[which][wave][u]);
+            ti[u] = t & 255;
+            tj[u] = t >> 8;
+            const unsigned km = kmask ? uni(kmask[ti[u]]) : dense;
+            mk[u] = uni(live ? km : 0u);
             acc[u] = d4{0.0, 0.0, 0.0, 0.0};
         }
         // operands of slab ks+1 are loaded while slab ks multiplies (all loads stay
@@ -403,21 +426,32 @@ struct IslandSolver {
                 if (mk[u] & (1u << ks)) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u], acc[u], 0, 0, 0);
         }
         if (in_place) __syncthreads();
+        // epilogue; off the diagonal of an unpadded matrix T = −R and ‖I − R‖² takes v²
+        const bool full = n == n16;
 #pragma unroll
         for (int u = 0; u < PER_WAVE; ++u) {
-            if (wave + u * (NTH / 64) >= T * T) continue;
+            if (u >= ntiles) continue;
+            const bool plain = full && ti[u] != tj[u];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int r = ti[u] * 16 + lk + 4 * j, c = tj[u] * 16 + lr;
-                const bool real = r < n && c < n;
                 double v = acc[u][j];
-                if (make_t) {
-                    const double d = (r == c) ? 1.0 : 0.0;
-                    const double x = d - v;
-                    part += real ? x * x : 0.0;
-                    v = 2.0 * d - v;
+                if (plain) {
+                    if (make_t) {
+                        part = fma(v, v, part);
+                        v = -v;
+                    }
+                } else {
+                    const bool real = r < n && c < n;
+                    if (make_t) {
+                        const double d = (r == c) ? 1.0 : 0.0;
+                        const double x = d - v;
+                        part += real ? x * x : 0.0;
+                        v = 2.0 * d - v;
+                    }
+                    v = real ? v : 0.0;
                 }
-                out[r * LD + c] = real ? v : 0.0;
+                out[r * LD + c] = v;
             }
         }
         return part;
@@ -438,9 +472,9 @@ struct IslandSolver {
         double before;
         if (mfma) {
             // T = 2I − A·X (A block-sparse: zero slabs skipped), then X ← X·T in place
-            double part = mfma_product(A(), X(), R(), s.kmask, true, false);
+            double part = mfma_product(A(), X(), R(), s.kmask, true, false, 0);
             before = block_sum(s, part);  // its barrier also publishes T
-            mfma_product(X(), R(), X(), nullptr, false, true);
+            mfma_product(X(), R(), X(), nullptr, false, true, 1);
         } else {
             gemm(A(), X(), R(), false, true, false);
             __syncthreads();
@@ -503,8 +537,28 @@ struct IslandSolver {
         }
     }
     // ───── dv = 0·dv + Σ_j X[i,j]·rhs_j ─────
+    // MFMA engine: four lanes per row, each summing every fourth block, combined by
+    // two butterfly steps (all lanes of a wave take part in the shuffles). The ordered
+    // engine keeps one lane per row and the reference's block order.
     __device__ void matvec() {
         const double* Xm = X();
+        static_assert(4 * 6 * MAXM <= NTH, "four lanes per row of the largest island");
+        if (mfma) {
+            const int t = threadIdx.x, row = t >> 2, part = t & 3;
+            const bool live = t < 4 * n;
+            double sum = 0.0;
+            if (live)
+                for (int j = part; j < m; j += 4) {
+                    const double* xb = Xm + row * LD + 6 * j;
+                    double sb = 0.0;
+                    for (int c = 0; c < 6; ++c) sb = sb + xb[c] * s.rhs0[j][c];
+                    sum = sum + sb;
+                }
+            sum += __shfl_xor(sum, 1, 64);
+            sum += __shfl_xor(sum, 2, 64);
+            if (live && part == 0) s.dv[row / 6][row % 6] = 0.0 * s.dv[row / 6][row % 6] + sum;
+            return;
+        }
         for (int t = threadIdx.x; t < n; t += NTH) {
             int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
@@ -779,61 +833,30 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     __syncthreads();
     for (int e = threadIdx.x; e < n * n; e += NTH) s.X[(e / n) * LD + e % n] = a.xacc[xoff + e];
     __syncthreads();
-    // ── topology tables: neighbours, assembly terms, incidence (joint order) ──
-    if (threadIdx.x == 0) {
-        int cnt[MAXM] = {};
-        for (int i = 0; i < m; ++i) { s.nbr[i * MAXM] = i; cnt[i] = 1; }
-        for (int j = 0; j < J; ++j) {
-            int ia = s.ja[j], ib = s.jb[j];
-            bool f = false;
-            for (int q = 0; q < cnt[ia]; ++q) f |= s.nbr[ia * MAXM + q] == ib;
-            if (!f) s.nbr[ia * MAXM + cnt[ia]++] = ib;
-            f = false;
-            for (int q = 0; q < cnt[ib]; ++q) f |= s.nbr[ib * MAXM + q] == ia;
-            if (!f) s.nbr[ib * MAXM + cnt[ib]++] = ia;
+    // ── topology tables (built on the host from the joint graph, one record per island) ──
+    {
+        using SH = IslandShared<MAXM, MAXJ, NTH, JF>;
+        static_assert(MAXM <= TOPO_MAXM && MAXJ <= TOPO_MAXJ && SH::WAVES <= TOPO_WAVES &&
+                          SH::PER_WAVE <= TOPO_PER_WAVE && SH::MAXT <= TOPO_MAXT,
+                      "island class exceeds the topology record");
+        const IslandTopo& tp = a.topo[isl];
+        const int t = threadIdx.x;
+        if (t <= MAXM) {
+            s.nbr_start[t] = tp.nbr_start[t];
+            s.inc_start[t] = tp.inc_start[t];
         }
-        // compact the neighbour lists
-        int o = 0;
-        int tmp[MAXM * MAXM];
-        for (int i = 0; i < m; ++i) {
-            s.nbr_start[i] = o;
-            for (int q = 0; q < cnt[i]; ++q) tmp[o++] = s.nbr[i * MAXM + q];
+        if (t < MAXM * MAXM) s.nbr[t] = tp.nbr[t];
+        if (t <= MAXM * MAXM) s.term_start[t] = tp.term_start[t];
+        if (t < 4 * SH::JD) s.terms[t] = tp.terms[t];
+        if (t < 2 * SH::JD) s.inc[t] = tp.inc[t];
+        if (t < SH::JD) s.jorder[t] = tp.jorder[t];
+        if (t < SH::MAXT) s.kmask[t] = tp.kmask[t];
+        if (t < 2 * SH::WAVES * SH::PER_WAVE) {
+            const int k = t / (SH::WAVES * SH::PER_WAVE), w = (t / SH::PER_WAVE) % SH::WAVES, u = t % SH::PER_WAVE;
+            const int e = tp.tlist[k][w][u];
+            s.tlist[k][w][u] = (e & 15) | ((e >> 4) << 8);   // row tile | column tile << 8
+            if (u == 0) s.tcount[k][w] = tp.tcount[k][w];
         }
-        s.nbr_start[m] = o;
-        for (int q = 0; q < o; ++q) s.nbr[q] = tmp[q];
-        o = 0;
-        for (int b = 0; b < m * m; ++b) {
-            s.term_start[b] = o;
-            int bi = b / m, bj = b % m;
-            for (int j = 0; j < J; ++j) {
-                int ends[2] = {s.ja[j], s.jb[j]};
-                for (int re = 0; re < 2; ++re)
-                    for (int ce = 0; ce < 2; ++ce)
-                        if (ends[re] == bi && ends[ce] == bj) s.terms[o++] = (j << 2) | (re << 1) | ce;
-            }
-        }
-        s.term_start[m * m] = o;
-        o = 0;
-        for (int i = 0; i < m; ++i) {
-            s.inc_start[i] = o;
-            for (int j = 0; j < J; ++j) {
-                if (s.ja[j] == i) s.inc[o++] = (j << 1) | 0;
-                if (s.jb[j] == i) s.inc[o++] = (j << 1) | 1;
-            }
-        }
-        s.inc_start[m] = o;
-        o = 0;
-        for (int k = 0; k < 4; ++k)
-            for (int j = 0; j < J; ++j)
-                if (s.jkind[j] == k) s.jorder[o++] = j;
-        // A's non-zero 4-column slabs per 16-row tile, from the block neighbour lists
-        for (int t = 0; t < IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 16; ++t) s.kmask[t] = 0u;
-        for (int bi = 0; bi < m; ++bi)
-            for (int q = s.nbr_start[bi]; q < s.nbr_start[bi + 1]; ++q) {
-                int bk = s.nbr[q];
-                for (int t = (6 * bi) / 16; t <= (6 * bi + 5) / 16; ++t)
-                    for (int ks = (6 * bk) / 4; ks <= (6 * bk + 5) / 4; ++ks) s.kmask[t] |= 1u << ks;
-            }
     }
     __syncthreads();
 
@@ -945,6 +968,25 @@ int island_class(int m, int j) {
     if (m <= 8 && j <= 14) return ISLAND_CLASS_M8J14;
     if (m <= 8 && j <= 16) return ISLAND_CLASS_M8J16;
     return -1;
+}
+
+void island_class_shape(int cls, int& waves, int& per_wave) {
+    switch (cls) {
+        case ISLAND_CLASS_M4J8:
+            waves = IslandShared<4, 8, 128, false>::WAVES;
+            per_wave = IslandShared<4, 8, 128, false>::PER_WAVE;
+            break;
+        case ISLAND_CLASS_M8J14:
+            waves = IslandShared<8, 14, 256, false>::WAVES;
+            per_wave = IslandShared<8, 14, 256, false>::PER_WAVE;
+            break;
+        case ISLAND_CLASS_M8J16:
+            waves = IslandShared<8, 16, 256, false>::WAVES;
+            per_wave = IslandShared<8, 16, 256, false>::PER_WAVE;
+            break;
+        default:
+            waves = per_wave = 1;
+    }
 }
 
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream) {

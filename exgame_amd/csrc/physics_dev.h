@@ -161,88 +161,116 @@ __host__ __device__ inline void body_position(const S pose[8], S pos[3]) {
 }
 
 // ───────────────────────── joints ─────────────────────────
+// A value of the pose type as a value of the velocity type: identity when they are
+// the same type, a constant (zero tangent) when the pose part carries no tangent.
+template <class SV, class SP>
+__host__ __device__ inline SV promote(const SP& x) {
+    return SV(x);
+}
+template <>
+__host__ __device__ inline Dual<double> promote<Dual<double>, double>(const double& x) {
+    return Dual<double>(x);
+}
+template <>
+__host__ __device__ inline Dual<float> promote<Dual<float>, float>(const float& x) {
+    return Dual<float>(x);
+}
+
 // Wrenches [on a, on b] at world poses and world velocities (a pure world function).
 // `params` in the canonical order of each joint (joints/src/lib.rs:273-299).
-template <class S>
-__host__ __device__ inline void joint_wrench(int kind, const double* params, const S pa_pose[8], const S pb_pose[8],
-                                             const S va[6], const S vb[6], double dt, double warp, S wa_out[6],
-                                             S wb_out[6]) {
-    using R = typename RealOf<S>::type;
-    S vrel[6];
+// The geometry (poses) is computed in SP and the velocity-dependent part in SV; with
+// SP = SV this is one type throughout. A Jacobian column along a velocity axis needs
+// tangents only in SV, so its geometry runs on plain values.
+template <class SP, class SV>
+__host__ __device__ inline void joint_wrench2(int kind, const double* params, const SP pa_pose[8],
+                                              const SP pb_pose[8], const SV va[6], const SV vb[6], double dt,
+                                              double warp, SV wa_out[6], SV wb_out[6]) {
+    using RP = typename RealOf<SP>::type;
+    using RV = typename RealOf<SV>::type;
+    SV vrel[6];
     for (int i = 0; i < 6; ++i) vrel[i] = va[i] - vb[i];
     if (kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED) {
-        S la[3] = {S(R(params[0])), S(R(params[1])), S(R(params[2]))};
-        S lb[3] = {S(R(params[3])), S(R(params[4])), S(R(params[5]))};
-        S wa, qa[3], wb, qb[3];
+        SP la[3] = {SP(RP(params[0])), SP(RP(params[1])), SP(RP(params[2]))};
+        SP lb[3] = {SP(RP(params[3])), SP(RP(params[4])), SP(RP(params[5]))};
+        SP wa, qa[3], wb, qb[3];
         point_sandwich(pa_pose, la, wa, qa);
         point_sandwich(pb_pose, lb, wb, qb);
-        S l[6];
+        SP l[6];
         join_points(wa, qa, wb, qb, l);
-        S eps = S(R(params[9]));
-        S d = s_sqrt(line_norm2(l) + eps * eps);
-        S raw[6];
+        SP eps = SP(RP(params[9]));
+        SP dp = s_sqrt(line_norm2(l) + eps * eps);
+        SP raw[6];
         wrench_from_line(l, raw);
-        S inv_d = cst<S>(1.0) / d;
-        S unit[6];
-        for (int i = 0; i < 6; ++i) unit[i] = raw[i] * inv_d;
-        S d_dot = pairing(unit, vrel);
-        S rest = S(R(params[6])), k = S(R(params[7])), c = S(R(params[8]));
-        S f;
+        SP inv_d = cst<SP>(1.0) / dp;
+        SV unit[6];
+        for (int i = 0; i < 6; ++i) unit[i] = promote<SV>(raw[i] * inv_d);
+        SV d = promote<SV>(dp);
+        SV d_dot = pairing(unit, vrel);
+        SV rest = SV(RV(params[6])), k = SV(RV(params[7])), c = SV(RV(params[8]));
+        SV f;
         if (kind == SIMPLE_SPRING_DAMPER) {
             f = k * (d - rest) + c * d_dot;
         } else {
-            S wp = S(R(warp));
-            S c0 = k / (wp * wp);
-            S omega_0 = s_sqrt(c0);
-            S dts = S(R(dt));
-            S den = cst<S>(1.0) + omega_0 * dts;
-            f = (c0 * (d - rest) + (cst<S>(1.0) + cst<S>(1.0)) * omega_0 * d_dot + c0 * dts * d_dot) / (den * den);
+            SV wp = SV(RV(warp));
+            SV c0 = k / (wp * wp);
+            SV omega_0 = s_sqrt(c0);
+            SV dts = SV(RV(dt));
+            SV den = cst<SV>(1.0) + omega_0 * dts;
+            f = (c0 * (d - rest) + (cst<SV>(1.0) + cst<SV>(1.0)) * omega_0 * d_dot + c0 * dts * d_dot) / (den * den);
         }
-        S nf = -f;
+        SV nf = -f;
         for (int i = 0; i < 6; ++i) {
             wa_out[i] = unit[i] * nf;
             wb_out[i] = -wa_out[i];
         }
     } else if (kind == PERPENDICULAR_DAMPER_WARPED) {
-        S z[3] = {cst<S>(0.0), cst<S>(0.0), cst<S>(0.0)};
-        S wa, qa[3], wb, qb[3];
-        point_sandwich(pa_pose, z, wa, qa);
+        SP z[3] = {cst<SP>(0.0), cst<SP>(0.0), cst<SP>(0.0)};
+        SP wap, qap[3], wb, qb[3];
+        point_sandwich(pa_pose, z, wap, qap);
         point_sandwich(pb_pose, z, wb, qb);
-        S l[6];
-        join_points(wa, qa, wb, qb, l);
-        S eps = S(R(params[1]));
-        S d = s_sqrt(line_norm2(l) + eps * eps);
-        S wp = S(R(warp));
-        S b_eff = S(R(params[0])) / (wp * wp);
-        S dir[3];
+        SP l[6];
+        join_points(wap, qap, wb, qb, l);
+        SP eps = SP(RP(params[1]));
+        SP dp = s_sqrt(line_norm2(l) + eps * eps);
+        SV wp = SV(RV(warp));
+        SV b_eff = SV(RV(params[0])) / (wp * wp);
+        SP dir[3];
         line_direction(l, dir);
-        S inv_d = cst<S>(1.0) / d;
-        S n[3] = {dir[0] * inv_d, dir[1] * inv_d, dir[2] * inv_d};
-        S v_rel[3];
+        SP inv_d = cst<SP>(1.0) / dp;
+        SV n[3] = {promote<SV>(dir[0] * inv_d), promote<SV>(dir[1] * inv_d), promote<SV>(dir[2] * inv_d)};
+        SV wa = promote<SV>(wap), qa[3] = {promote<SV>(qap[0]), promote<SV>(qap[1]), promote<SV>(qap[2])};
+        SV v_rel[3];
         velocity_at(vrel, wa, qa, v_rel);
-        S v_dot_n = v_rel[0] * n[0] + (v_rel[1] * n[1] + (v_rel[2] * n[2] + cst<S>(0.0)));
-        S nb = -b_eff;
-        S winv = cst<S>(1.0) / wa;
-        S tip[3];
+        SV v_dot_n = v_rel[0] * n[0] + (v_rel[1] * n[1] + (v_rel[2] * n[2] + cst<SV>(0.0)));
+        SV nb = -b_eff;
+        SV winv = cst<SV>(1.0) / wa;
+        SV tip[3];
         for (int i = 0; i < 3; ++i) {
-            S v_perp = v_rel[i] - n[i] * v_dot_n;
-            S f = v_perp * nb;
+            SV v_perp = v_rel[i] - n[i] * v_dot_n;
+            SV f = v_perp * nb;
             tip[i] = qa[i] * winv + f;
         }
-        S l2[6];
-        join_points(cst<S>(1.0), tip, wa, qa, l2);
+        SV l2[6];
+        join_points(cst<SV>(1.0), tip, wa, qa, l2);
         wrench_from_line(l2, wa_out);
         for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
     } else {
-        S wp = S(R(warp));
-        S b_eff = S(R(params[0])) / (wp * wp);
-        S nb = -b_eff;
+        SV wp = SV(RV(warp));
+        SV b_eff = SV(RV(params[0])) / (wp * wp);
+        SV nb = -b_eff;
         for (int i = 0; i < 3; ++i) {
-            wa_out[i] = cst<S>(0.0);
+            wa_out[i] = cst<SV>(0.0);
             wa_out[3 + i] = vrel[3 + i] * nb;
         }
         for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
     }
+}
+
+template <class S>
+__host__ __device__ inline void joint_wrench(int kind, const double* params, const S pa_pose[8], const S pb_pose[8],
+                                             const S va[6], const S vb[6], double dt, double warp, S wa_out[6],
+                                             S wb_out[6]) {
+    joint_wrench2<S, S>(kind, params, pa_pose, pb_pose, va, vb, dt, warp, wa_out, wb_out);
 }
 
 // Derivative seeds of axis `axis` (0..23) for the unit retraction base∘exp(δ):

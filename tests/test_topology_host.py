@@ -1,0 +1,164 @@
+"""The per-island topology record the Newton kernels load every step
+(exgame_amd/csrc/topology.cpp), built on the host and checked against what the joint
+graph fixes: neighbour sets (the block sparsity of A), every (joint, end, end) term
+and (joint, end) incidence exactly once, joints grouped by kind, slab masks equal to
+the non-zero pattern of A, and MFMA tile lists that cover every output tile once."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "_build", "libtopo.so")
+SRC = os.path.join(HERE, "csrc", "topo_host.cpp")
+DEPS = [SRC] + [os.path.join(HERE, "..", "exgame_amd", "csrc", f) for f in ("topology.cpp", "island.h")]
+
+# record layout (island.h, struct IslandTopo)
+MAXM, MAXJ, WAVES, PER_WAVE = 8, 16, 4, 4
+FIELDS = [("kmask", "<u2", 4), ("tcount", "u1", 2 * WAVES), ("tlist", "u1", 2 * WAVES * PER_WAVE),
+          ("nbr_start", "u1", MAXM + 1), ("nbr", "u1", MAXM * MAXM), ("term_start", "u1", MAXM * MAXM + 1),
+          ("terms", "u1", 4 * MAXJ), ("inc_start", "u1", MAXM + 1), ("inc", "u1", 2 * MAXJ), ("jorder", "u1", MAXJ)]
+
+
+@pytest.fixture(scope="module")
+def tp():
+    if not os.path.exists(SO) or any(os.path.getmtime(d) > os.path.getmtime(SO) for d in DEPS):
+        os.makedirs(os.path.dirname(SO), exist_ok=True)
+        subprocess.run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared", SRC,
+                        "-o", SO], check=True)
+    L = C.CDLL(SO)
+    assert L.tp_size() == 320
+    return L
+
+
+def build(L, m, ja, jb, kind, waves, per_wave):
+    J = len(ja)
+    buf = np.zeros(320, dtype=np.uint8)
+    arr = [np.ascontiguousarray(x, dtype=np.int32) for x in (ja, jb, kind)]
+    ip = C.POINTER(C.c_int)
+    rc = L.tp_build(m, J, *[x.ctypes.data_as(ip) for x in arr], waves, per_wave,
+                    buf.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    out, off = {}, 0
+    for name, dt, cnt in FIELDS:
+        size = np.dtype(dt).itemsize * cnt
+        out[name] = np.frombuffer(buf[off:off + size].tobytes(), dtype=dt).astype(np.int64)
+        off += size
+    out["tcount"] = out["tcount"].reshape(2, WAVES)
+    out["tlist"] = out["tlist"].reshape(2, WAVES, PER_WAVE)
+    return rc, out
+
+
+def shape_of(m, J):
+    return (2, 2) if (m <= 4 and J <= 8) else (4, 3)
+
+
+def graphs():
+    rng = np.random.default_rng(7)
+    out = []
+    for m in range(2, 9):   # chains
+        ja = list(range(m - 1)); jb = list(range(1, m))
+        out.append((m, ja, jb, [i % 4 for i in range(m - 1)]))
+        # chain with a torsional damper per link (two joints per pair) and braces
+        ja2 = ja + ja + [i for i in range(m - 2)]
+        jb2 = jb + jb + [i + 2 for i in range(m - 2)]
+        if len(ja2) <= MAXJ:
+            out.append((m, ja2, jb2, list(rng.integers(0, 4, len(ja2)))))
+    out.append((4, [0, 0, 0, 1, 2], [1, 2, 3, 2, 3], [0, 1, 2, 3, 0]))   # a 4-body rig
+    for _ in range(40):   # random connected graphs
+        m = int(rng.integers(2, 9))
+        J = int(rng.integers(m - 1, MAXJ + 1))
+        ja, jb = [], []
+        for i in range(1, m):   # spanning tree first
+            p = int(rng.integers(0, i))
+            if rng.random() < 0.5: ja.append(p); jb.append(i)
+            else: ja.append(i); jb.append(p)
+        while len(ja) < J:
+            x, y = rng.choice(m, 2, replace=False)
+            ja.append(int(x)); jb.append(int(y))
+        out.append((m, ja, jb, list(rng.integers(0, 4, J))))
+    return out
+
+
+@pytest.mark.parametrize("case", range(len(graphs())))
+def test_topology_record(tp, case):
+    m, ja, jb, kind = graphs()[case]
+    J = len(ja)
+    waves, per_wave = shape_of(m, J)
+    rc, t = build(tp, m, ja, jb, kind, waves, per_wave)
+    assert rc == 0
+    # neighbour lists: the body itself first, then the other ends in joint order, once each
+    for i in range(m):
+        lst = list(t["nbr"][t["nbr_start"][i]:t["nbr_start"][i + 1]])
+        want = [i]
+        for a, b in zip(ja, jb):
+            for me, other in ((a, b), (b, a)):
+                if me == i and other not in want:
+                    want.append(other)
+        assert lst == want
+    # every (joint, row end, column end) exactly once, filed under its block, in joint order
+    seen = []
+    for blk in range(m * m):
+        bi, bj = divmod(blk, m)
+        terms = list(t["terms"][t["term_start"][blk]:t["term_start"][blk + 1]])
+        assert terms == sorted(terms)
+        for e in terms:
+            j, re, ce = e >> 2, (e >> 1) & 1, e & 1
+            ends = (ja[j], jb[j])
+            assert ends[re] == bi and ends[ce] == bj
+            seen.append(e)
+    assert sorted(seen) == [(j << 2) | k for j in range(J) for k in range(4)]
+    # incidence: every (joint, end) once, under the body at that end
+    seen = []
+    for i in range(m):
+        for e in t["inc"][t["inc_start"][i]:t["inc_start"][i + 1]]:
+            j, end = e >> 1, e & 1
+            assert (ja[j], jb[j])[end] == i
+            seen.append(e)
+    assert sorted(seen) == list(range(2 * J))
+    # joints grouped by kind, stable within a kind
+    order = list(t["jorder"][:J])
+    assert sorted(order) == list(range(J))
+    assert order == sorted(range(J), key=lambda j: (kind[j], j))
+    # slab masks equal the non-zero pattern of A (6×6 blocks of coupled bodies)
+    n = 6 * m
+    T = (n + 15) // 16
+    nz = np.zeros((16 * T, 16 * T), dtype=bool)
+    for i in range(m):
+        nz[6 * i:6 * i + 6, 6 * i:6 * i + 6] = True
+    for a, b in zip(ja, jb):
+        nz[6 * a:6 * a + 6, 6 * b:6 * b + 6] = True
+        nz[6 * b:6 * b + 6, 6 * a:6 * a + 6] = True
+    for ti in range(T):
+        want = 0
+        for ks in range(4 * T):
+            if nz[16 * ti:16 * ti + 16, 4 * ks:4 * ks + 4].any():
+                want |= 1 << ks
+        assert t["kmask"][ti] == want, (ti, bin(t["kmask"][ti]), bin(want))
+    # tile lists: each of the T² output tiles once per product, within the wave slots
+    for k in range(2):
+        tiles = []
+        for w in range(WAVES):
+            c = t["tcount"][k][w]
+            assert c <= (per_wave if w < waves else 0)
+            for u in range(c):
+                e = t["tlist"][k][w][u]
+                ti, tj = e & 15, e >> 4
+                assert ti < T and tj < T
+                tiles.append(ti * T + tj)
+        assert sorted(tiles) == list(range(T * T))
+    # the sparse product's lists: no wave carries more non-zero slabs than the
+    # largest-first bound (mean load + the largest single tile)
+    cost = [bin(int(t["kmask"][x // T])).count("1") for x in range(T * T)]
+    loads = [sum(cost[(e & 15) * T + (e >> 4)] for e in t["tlist"][0][w][:t["tcount"][0][w]]) for w in range(waves)]
+    assert max(loads) <= sum(cost) / waves + max(cost)
+
+
+def test_topology_rejects_oversize(tp):
+    rc, _ = build(tp, 9, [0], [1], [0], 4, 3)
+    assert rc == -1
+    rc, _ = build(tp, 2, [0] * 17, [1] * 17, [0] * 17, 4, 3)
+    assert rc == -1
+    rc, _ = build(tp, 3, [0], [3], [0], 4, 3)   # end outside the island
+    assert rc == -1
