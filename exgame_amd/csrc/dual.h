@@ -30,6 +30,15 @@ template <class T> __host__ __device__ inline Dual<T> recip(const Dual<T>& b) {
 }
 template <class T> __host__ __device__ inline Dual<T> operator/(const Dual<T>& a, const Dual<T>& b) { return a * recip(b); }
 
+// Mixed dual × constant forms (a constant has no tangent): the same values and
+// derivatives as promoting the constant to Dual(c, 0), without the products by zero.
+template <class T> __host__ __device__ inline Dual<T> operator*(const Dual<T>& a, T b) { return {a.v * b, a.d * b}; }
+template <class T> __host__ __device__ inline Dual<T> operator*(T a, const Dual<T>& b) { return {a * b.v, b.d * a}; }
+template <class T> __host__ __device__ inline Dual<T> operator+(const Dual<T>& a, T b) { return {a.v + b, a.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator+(T a, const Dual<T>& b) { return {a + b.v, b.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator-(const Dual<T>& a, T b) { return {a.v - b, a.d}; }
+template <class T> __host__ __device__ inline Dual<T> operator-(T a, const Dual<T>& b) { return {a - b.v, -b.d}; }
+
 template <class T> __host__ __device__ inline T value_of(const Dual<T>& x) { return x.v; }
 __host__ __device__ inline double value_of(double x) { return x; }
 __host__ __device__ inline float value_of(float x) { return x; }

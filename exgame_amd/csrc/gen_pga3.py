@@ -182,8 +182,10 @@ class Emitter:
     def __init__(self):
         self.lines = []
 
-    def fn(self, sig, body_lines):
-        self.lines.append("template <class S>")
+    def fn(self, sig, body_lines, tmpl="template <class S>", comment=None):
+        if comment:
+            self.lines.extend("// " + c for c in comment)
+        self.lines.append(tmpl)
         self.lines.append(f"__host__ __device__ inline {sig} {{")
         for l in body_lines:
             self.lines.append("    " + l)
@@ -278,7 +280,10 @@ def main():
         s = poly_c(q[i])
         s = sub_index(sub_index(s, "m", 8), "p", 3)
         body.append(f"q[{i}] = {s};")
-    em.fn("void point_sandwich(const S m[8], const S p[3], S& w, S q[3])", body)
+    body = ["using R = typename RealOf<S>::type;"] + [b.replace("S(2.0)", "R(2.0)") for b in body]
+    em.fn("void point_sandwich(const S m[8], const P p[3], S& w, S q[3])", body, tmpl="template <class S, class P = S>",
+          comment=["The point p may be of a plain type P while the motor carries tangents (S = Dual):",
+                   "its terms then multiply by a constant."])
 
     # join of two points (weights wa, wb; coords qa, qb) → line bivector (6 grade-2 comps, ascending blade)
     wa, wb = sp.symbols("wa wb")
@@ -352,6 +357,7 @@ def main():
         "// Closed-form PGA3 operations (exact polynomial expansions of the reference's",
         "// multivector expressions) for the device kernels, templated on the scalar.",
         "#pragma once",
+        '#include "dual.h"',
         "",
         "namespace exg {",
         "",

@@ -123,49 +123,6 @@ __device__ double block_sum(IslandShared<MAXM, MAXJ, NTH, JF>& s, double v) {
     return uni(r);
 }
 
-// One Jacobian column of a joint (forward-mode tangents along one axis) and, on
-// request, the wrench values. Axes 0-5 / 6-11 perturb the pose of end a / b (unit
-// left retraction), axes 12-17 / 18-23 its velocity. All pointers address LDS.
-// Inlined: an out-of-line version, storing its results to LDS through flat
-// pointers from inside the call, gave wrong solves in one kernel instantiation
-// (those stores were not reliably visible after the caller's barrier).
-// T = float is the mixed-precision Jacobian: inputs narrowed to f32, f32 tangents.
-template <class T>
-__device__ __forceinline__ void joint_column(int kind, const double* params, const double* pose_a,
-                                          const double* pose_b, const double* vel_a, const double* vel_b, int ax,
-                                          double dt, double warp, T* da, T* db, double* va, double* vb) {
-    using D = Dual<T>;
-    D vv[2][6], wa[6], wb[6];
-    const double* poses[2] = {pose_a, pose_b};
-    const double* vels[2] = {vel_a, vel_b};
-    for (int e = 0; e < 2; ++e)
-        for (int k = 0; k < 6; ++k) vv[e][k] = D((T)vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? T(1) : T(0));
-    if (ax < 12) {  // pose axis: tangents through the geometry
-        D pp[2][8];
-        for (int e = 0; e < 2; ++e) {
-            T base[8];
-            for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
-            if (ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-        }
-        joint_wrench2<D, D>(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
-    } else {  // velocity axis: the geometry carries no tangent
-        T pz[2][8];
-        for (int e = 0; e < 2; ++e)
-            for (int k = 0; k < 8; ++k) pz[e][k] = (T)poses[e][k];
-        joint_wrench2<T, D>(kind, params, pz[0], pz[1], vv[0], vv[1], dt, warp, wa, wb);
-    }
-    for (int k = 0; k < 6; ++k) {
-        da[k] = wa[k].d;
-        db[k] = wb[k].d;
-    }
-    if (va)
-        for (int k = 0; k < 6; ++k) {
-            va[k] = wa[k].v;
-            vb[k] = wb[k].v;
-        }
-}
-
 template <int MAXM, int MAXJ, int NTH, bool JF>
 struct IslandSolver {
     IslandShared<MAXM, MAXJ, NTH, JF>& s;

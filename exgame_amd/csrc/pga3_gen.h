@@ -2,6 +2,7 @@
 // Closed-form PGA3 operations (exact polynomial expansions of the reference's
 // multivector expressions) for the device kernels, templated on the scalar.
 #pragma once
+#include "dual.h"
 
 namespace exg {
 
@@ -109,12 +110,15 @@ __host__ __device__ inline void coad_matrix(const S m[8], S cad[6][6]) {
     cad[5][5] = m[0]*m[0] + m[1]*m[1] - m[2]*m[2] - m[3]*m[3];
 }
 
-template <class S>
-__host__ __device__ inline void point_sandwich(const S m[8], const S p[3], S& w, S q[3]) {
+// The point p may be of a plain type P while the motor carries tangents (S = Dual):
+// its terms then multiply by a constant.
+template <class S, class P = S>
+__host__ __device__ inline void point_sandwich(const S m[8], const P p[3], S& w, S q[3]) {
+    using R = typename RealOf<S>::type;
     w = m[0]*m[0] + m[1]*m[1] + m[2]*m[2] + m[3]*m[3];
-    q[0] = p[0]*m[0]*m[0] + p[0]*m[3]*m[3] - p[0]*m[1]*m[1] - p[0]*m[2]*m[2] - S(2.0)*m[0]*m[4] - S(2.0)*m[1]*m[5] - S(2.0)*m[2]*m[6] - S(2.0)*m[3]*m[7] - S(2.0)*m[2]*m[3]*p[1] + S(2.0)*m[0]*m[1]*p[1] + S(2.0)*m[0]*m[2]*p[2] + S(2.0)*m[1]*m[3]*p[2];
-    q[1] = p[1]*m[0]*m[0] + p[1]*m[2]*m[2] - p[1]*m[1]*m[1] - p[1]*m[3]*m[3] - S(2.0)*m[0]*m[5] - S(2.0)*m[3]*m[6] + S(2.0)*m[1]*m[4] + S(2.0)*m[2]*m[7] - S(2.0)*m[0]*m[1]*p[0] - S(2.0)*m[1]*m[2]*p[2] - S(2.0)*m[2]*m[3]*p[0] + S(2.0)*m[0]*m[3]*p[2];
-    q[2] = p[2]*m[0]*m[0] + p[2]*m[1]*m[1] - p[2]*m[2]*m[2] - p[2]*m[3]*m[3] - S(2.0)*m[0]*m[6] - S(2.0)*m[1]*m[7] + S(2.0)*m[2]*m[4] + S(2.0)*m[3]*m[5] - S(2.0)*m[0]*m[2]*p[0] - S(2.0)*m[0]*m[3]*p[1] - S(2.0)*m[1]*m[2]*p[1] + S(2.0)*m[1]*m[3]*p[0];
+    q[0] = p[0]*m[0]*m[0] + p[0]*m[3]*m[3] - p[0]*m[1]*m[1] - p[0]*m[2]*m[2] - R(2.0)*m[0]*m[4] - R(2.0)*m[1]*m[5] - R(2.0)*m[2]*m[6] - R(2.0)*m[3]*m[7] - R(2.0)*m[2]*m[3]*p[1] + R(2.0)*m[0]*m[1]*p[1] + R(2.0)*m[0]*m[2]*p[2] + R(2.0)*m[1]*m[3]*p[2];
+    q[1] = p[1]*m[0]*m[0] + p[1]*m[2]*m[2] - p[1]*m[1]*m[1] - p[1]*m[3]*m[3] - R(2.0)*m[0]*m[5] - R(2.0)*m[3]*m[6] + R(2.0)*m[1]*m[4] + R(2.0)*m[2]*m[7] - R(2.0)*m[0]*m[1]*p[0] - R(2.0)*m[1]*m[2]*p[2] - R(2.0)*m[2]*m[3]*p[0] + R(2.0)*m[0]*m[3]*p[2];
+    q[2] = p[2]*m[0]*m[0] + p[2]*m[1]*m[1] - p[2]*m[2]*m[2] - p[2]*m[3]*m[3] - R(2.0)*m[0]*m[6] - R(2.0)*m[1]*m[7] + R(2.0)*m[2]*m[4] + R(2.0)*m[3]*m[5] - R(2.0)*m[0]*m[2]*p[0] - R(2.0)*m[0]*m[3]*p[1] - R(2.0)*m[1]*m[2]*p[1] + R(2.0)*m[1]*m[3]*p[0];
 }
 
 template <class S>

@@ -176,6 +176,46 @@ __host__ __device__ inline Dual<float> promote<Dual<float>, float>(const float& 
     return Dual<float>(x);
 }
 
+// The axial spring-damper's wrenches from its two world anchor points (homogeneous
+// weight and coordinates): line, soft length, unit wrench, length rate, force law
+// (axial_spring_damper/mod.rs:315-385; the laws at :47-61 and
+// critically_damped_warped.rs:58-90).
+template <class SP, class SV>
+__host__ __device__ inline void axial_from_points(int kind, const double* params, const SP& wa, const SP qa[3],
+                                                  const SP& wb, const SP qb[3], const SV vrel[6], double dt,
+                                                  double warp, SV wa_out[6], SV wb_out[6]) {
+    using RP = typename RealOf<SP>::type;
+    using RV = typename RealOf<SV>::type;
+    SP l[6];
+    join_points(wa, qa, wb, qb, l);
+    SP eps = SP(RP(params[9]));
+    SP dp = s_sqrt(line_norm2(l) + eps * eps);
+    SP raw[6];
+    wrench_from_line(l, raw);
+    SP inv_d = cst<SP>(1.0) / dp;
+    SV unit[6];
+    for (int i = 0; i < 6; ++i) unit[i] = promote<SV>(raw[i] * inv_d);
+    SV d = promote<SV>(dp);
+    SV d_dot = pairing(unit, vrel);
+    SV rest = SV(RV(params[6])), k = SV(RV(params[7])), c = SV(RV(params[8]));
+    SV f;
+    if (kind == SIMPLE_SPRING_DAMPER) {
+        f = k * (d - rest) + c * d_dot;
+    } else {
+        SV wp = SV(RV(warp));
+        SV c0 = k / (wp * wp);
+        SV omega_0 = s_sqrt(c0);
+        SV dts = SV(RV(dt));
+        SV den = cst<SV>(1.0) + omega_0 * dts;
+        f = (c0 * (d - rest) + (cst<SV>(1.0) + cst<SV>(1.0)) * omega_0 * d_dot + c0 * dts * d_dot) / (den * den);
+    }
+    SV nf = -f;
+    for (int i = 0; i < 6; ++i) {
+        wa_out[i] = unit[i] * nf;
+        wb_out[i] = -wa_out[i];
+    }
+}
+
 // Wrenches [on a, on b] at world poses and world velocities (a pure world function).
 // `params` in the canonical order of each joint (joints/src/lib.rs:273-299).
 // The geometry (poses) is computed in SP and the velocity-dependent part in SV; with
@@ -190,39 +230,12 @@ __host__ __device__ inline void joint_wrench2(int kind, const double* params, co
     SV vrel[6];
     for (int i = 0; i < 6; ++i) vrel[i] = va[i] - vb[i];
     if (kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED) {
-        SP la[3] = {SP(RP(params[0])), SP(RP(params[1])), SP(RP(params[2]))};
-        SP lb[3] = {SP(RP(params[3])), SP(RP(params[4])), SP(RP(params[5]))};
+        RP la[3] = {RP(params[0]), RP(params[1]), RP(params[2])};
+        RP lb[3] = {RP(params[3]), RP(params[4]), RP(params[5])};
         SP wa, qa[3], wb, qb[3];
         point_sandwich(pa_pose, la, wa, qa);
         point_sandwich(pb_pose, lb, wb, qb);
-        SP l[6];
-        join_points(wa, qa, wb, qb, l);
-        SP eps = SP(RP(params[9]));
-        SP dp = s_sqrt(line_norm2(l) + eps * eps);
-        SP raw[6];
-        wrench_from_line(l, raw);
-        SP inv_d = cst<SP>(1.0) / dp;
-        SV unit[6];
-        for (int i = 0; i < 6; ++i) unit[i] = promote<SV>(raw[i] * inv_d);
-        SV d = promote<SV>(dp);
-        SV d_dot = pairing(unit, vrel);
-        SV rest = SV(RV(params[6])), k = SV(RV(params[7])), c = SV(RV(params[8]));
-        SV f;
-        if (kind == SIMPLE_SPRING_DAMPER) {
-            f = k * (d - rest) + c * d_dot;
-        } else {
-            SV wp = SV(RV(warp));
-            SV c0 = k / (wp * wp);
-            SV omega_0 = s_sqrt(c0);
-            SV dts = SV(RV(dt));
-            SV den = cst<SV>(1.0) + omega_0 * dts;
-            f = (c0 * (d - rest) + (cst<SV>(1.0) + cst<SV>(1.0)) * omega_0 * d_dot + c0 * dts * d_dot) / (den * den);
-        }
-        SV nf = -f;
-        for (int i = 0; i < 6; ++i) {
-            wa_out[i] = unit[i] * nf;
-            wb_out[i] = -wa_out[i];
-        }
+        axial_from_points<SP, SV>(kind, params, wa, qa, wb, qb, vrel, dt, warp, wa_out, wb_out);
     } else if (kind == PERPENDICULAR_DAMPER_WARPED) {
         SP z[3] = {cst<SP>(0.0), cst<SP>(0.0), cst<SP>(0.0)};
         SP wap, qap[3], wb, qb[3];
@@ -285,6 +298,79 @@ __host__ __device__ inline void seed_pose(const T base[8], int dof, Dual<T> out[
     T d[8];
     motor_mul(bv, base, d);
     for (int i = 0; i < 8; ++i) out[i] = Dual<T>(base[i], d[i]);
+}
+
+// One Jacobian column of a joint (forward-mode tangents along one axis) and, on
+// request, the wrench values. Axes 0-5 / 6-11 perturb the pose of end a / b (unit
+// left retraction), axes 12-17 / 18-23 its velocity. In the island kernel all
+// pointers address LDS.
+// Inlined: an out-of-line version, storing its results to LDS through flat
+// pointers from inside the call, gave wrong solves in one kernel instantiation
+// (those stores were not reliably visible after the caller's barrier).
+// T = float is the mixed-precision Jacobian: inputs narrowed to f32, f32 tangents.
+template <class T>
+__host__ __device__ __forceinline__ void joint_column(int kind, const double* params, const double* pose_a,
+                                          const double* pose_b, const double* vel_a, const double* vel_b, int ax,
+                                          double dt, double warp, T* da, T* db, double* va, double* vb) {
+    using D = Dual<T>;
+    D vv[2][6], wa[6], wb[6];
+    const double* poses[2] = {pose_a, pose_b};
+    const double* vels[2] = {vel_a, vel_b};
+    for (int e = 0; e < 2; ++e)
+        for (int k = 0; k < 6; ++k) vv[e][k] = D((T)vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? T(1) : T(0));
+    if (ax < 12 && (kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED)) {
+        // pose axis of an axial joint: only the perturbed end's anchor carries a
+        // tangent; the other end's anchor is a plain sandwich (ends selected per lane)
+        const int e = ax / 6;
+        T sb[8], ob[8], sa[3], oa[3];
+        for (int k = 0; k < 8; ++k) {
+            sb[k] = (T)(e ? pose_b[k] : pose_a[k]);
+            ob[k] = (T)(e ? pose_a[k] : pose_b[k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            sa[k] = (T)params[e ? 3 + k : k];
+            oa[k] = (T)params[e ? k : 3 + k];
+        }
+        D ps[8];
+        seed_pose(sb, ax % 6, ps);
+        D ws, qs[3];
+        point_sandwich(ps, sa, ws, qs);
+        T wo, qo[3];
+        point_sandwich(ob, oa, wo, qo);
+        const D wod(wo);
+        D pa_w = e ? wod : ws, pb_w = e ? ws : wod, pa_q[3], pb_q[3];
+        for (int k = 0; k < 3; ++k) {
+            const D qk(qo[k]);
+            pa_q[k] = e ? qk : qs[k];
+            pb_q[k] = e ? qs[k] : qk;
+        }
+        D vrel[6];
+        for (int k = 0; k < 6; ++k) vrel[k] = vv[0][k] - vv[1][k];
+        axial_from_points<D, D>(kind, params, pa_w, pa_q, pb_w, pb_q, vrel, dt, warp, wa, wb);
+    } else if (ax < 12) {  // pose axis of another kind: tangents through the geometry
+        D pp[2][8];
+        for (int e = 0; e < 2; ++e) {
+            T base[8];
+            for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
+            if (ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
+            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
+        }
+        joint_wrench2<D, D>(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+    } else {  // velocity axis: the geometry carries no tangent
+        T pz[2][8];
+        for (int e = 0; e < 2; ++e)
+            for (int k = 0; k < 8; ++k) pz[e][k] = (T)poses[e][k];
+        joint_wrench2<T, D>(kind, params, pz[0], pz[1], vv[0], vv[1], dt, warp, wa, wb);
+    }
+    for (int k = 0; k < 6; ++k) {
+        da[k] = wa[k].d;
+        db[k] = wb[k].d;
+    }
+    if (va)
+        for (int k = 0; k < 6; ++k) {
+            va[k] = wa[k].v;
+            vb[k] = wb[k].v;
+        }
 }
 
 // ───────────────────────── body_post (functions/src/lib.rs:170-247) ─────────────────────────

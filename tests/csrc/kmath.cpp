@@ -29,27 +29,16 @@ void km_joint_plain(int kind, const double* params, const double* poses16, const
     joint_wrench(kind, params, poses16, poses16 + 8, vels12, vels12 + 6, dt, warp, out12, out12 + 6);
 }
 
-// Same seeding as the Jacobian kernel: one dual lane per axis.
+// The island kernel's Jacobian lanes (joint_column), one per axis.
 void km_joint_jacobian(int kind, const double* params, const double* poses16, const double* vels12, double dt,
                        double warp, double* value12, double* jac288) {
-    using D = Dual<double>;
     for (int ax = 0; ax < 24; ++ax) {
-        D pp[2][8], vv[2][6];
-        for (int e = 0; e < 2; ++e) {
-            const double* base = poses16 + 8 * e;
-            if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-            for (int k = 0; k < 6; ++k) vv[e][k] = D(vels12[6 * e + k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0 : 0.0);
-        }
-        D wa[6], wb[6];
-        joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+        double da[6], db[6];
+        joint_column<double>(kind, params, poses16, poses16 + 8, vels12, vels12 + 6, ax, dt, warp, da, db,
+                             ax == 0 ? value12 : nullptr, ax == 0 ? value12 + 6 : nullptr);
         for (int k = 0; k < 6; ++k) {
-            jac288[(0 * 24 + ax) * 6 + k] = wa[k].d;
-            jac288[(1 * 24 + ax) * 6 + k] = wb[k].d;
-            if (ax == 0) {
-                value12[k] = wa[k].v;
-                value12[6 + k] = wb[k].v;
-            }
+            jac288[(0 * 24 + ax) * 6 + k] = da[k];
+            jac288[(1 * 24 + ax) * 6 + k] = db[k];
         }
     }
 }
@@ -58,22 +47,13 @@ void km_joint_jacobian(int kind, const double* params, const double* poses16, co
 // to f32 (the island kernel's f32 Jacobian path), widened for comparison.
 void km_joint_jacobian_f32(int kind, const double* params, const double* poses16, const double* vels12, double dt,
                            double warp, double* jac288) {
-    using D = Dual<float>;
     for (int ax = 0; ax < 24; ++ax) {
-        D pp[2][8], vv[2][6];
-        for (int e = 0; e < 2; ++e) {
-            float base[8];
-            for (int k = 0; k < 8; ++k) base[k] = (float)poses16[8 * e + k];
-            if (ax < 12 && ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-            for (int k = 0; k < 6; ++k)
-                vv[e][k] = D((float)vels12[6 * e + k], (ax >= 12 && (ax - 12) == 6 * e + k) ? 1.0f : 0.0f);
-        }
-        D wa[6], wb[6];
-        joint_wrench(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
+        float da[6], db[6];
+        joint_column<float>(kind, params, poses16, poses16 + 8, vels12, vels12 + 6, ax, dt, warp, da, db, nullptr,
+                            nullptr);
         for (int k = 0; k < 6; ++k) {
-            jac288[(0 * 24 + ax) * 6 + k] = wa[k].d;
-            jac288[(1 * 24 + ax) * 6 + k] = wb[k].d;
+            jac288[(0 * 24 + ax) * 6 + k] = da[k];
+            jac288[(1 * 24 + ax) * 6 + k] = db[k];
         }
     }
 }
