@@ -56,7 +56,6 @@ struct LaneSlots {
     double total[LS - 1][MAXM][6];
     double rhs[LS - 1][MAXM][6];
     double scale[LS - 1][MAXM][2];
-    double ratio[LS - 1][MAXM][2];
     double conn[LS - 1][JD][2][6];
 };
 
@@ -88,7 +87,7 @@ struct IslandShared {
     static constexpr int PER_WAVE = (MAXT * MAXT + WAVES - 1) / WAVES;
     int tlist[2][WAVES][PER_WAVE];          // MFMA tiles of each wave: [0] sparse product, [1] dense
     int tcount[2][WAVES];
-    double vmid0[MAXM][6], mid0[MAXM][8], total0[MAXM][6], rhs0[MAXM][6], scale0[MAXM][2], ratio0[MAXM][2];
+    double vmid0[MAXM][6], mid0[MAXM][8], total0[MAXM][6], rhs0[MAXM][6], scale0[MAXM][2];
     double conn0[JD][2][6];
     double wn[LS];
     double mass0[MAXM][36];                 // candidate 0's world mass block
@@ -146,7 +145,6 @@ struct IslandSolver {
     __device__ double* TOTAL(int l, int i) { return l == 0 ? s.total0[i] : s.u.ls.total[sl(l)][i]; }
     __device__ double* RHS(int l, int i) { return l == 0 ? s.rhs0[i] : s.u.ls.rhs[sl(l)][i]; }
     __device__ double* SCALE(int l, int i) { return l == 0 ? s.scale0[i] : s.u.ls.scale[sl(l)][i]; }
-    __device__ double* RATIO(int l, int i) { return l == 0 ? s.ratio0[i] : s.u.ls.ratio[sl(l)][i]; }
     __device__ double* CONN(int l, int j, int e) { return l == 0 ? s.conn0[j][e] : s.u.ls.conn[sl(l)][j][e]; }
 
     // ───── PRE: midpoint = pose ∘ exp(half·vmid) for lanes [l0, l1) ─────
@@ -204,49 +202,51 @@ struct IslandSolver {
             for (int r = 0; r < 6; ++r)
                 for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = block[r][c];
     }
+    static __device__ double max2(double x, double y) { return x > y ? x : y; }
+    // Gather + body post of lanes [l0, l1), one thread per (lane, body), and the worst²
+    // residual ratio of each lane (newton.rs:139-151) into s.wn (visible after the
+    // caller's barrier). Every (lane, body) thread sits in wave 0: each forms its two
+    // ratios, and the lane's thread folds its bodies' ratios in body order, reading
+    // them across the wave with the same max2 (NaN handling as the sequential form).
     // Candidate 0's world mass block goes to mass0 only when mass0 may take it: in
     // the second line-search round candidate 0 is lane 4, and mass0 must keep lane
     // 0's block unless a later lane is accepted (the oracle's lanes[0].mass).
     __device__ void gather_post(int l0, int l1, bool mass_of_candidate0 = true) {
-        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NTH) {
-            int lane = l0 + t / m, i = t % m;
-            gather_post_one(lane, i, lane == 0 && mass_of_candidate0);
-        }
-    }
-    static __device__ double max2(double x, double y) { return x > y ? x : y; }
-    // ───── worst² residual ratio per lane (newton.rs:139-151) ─────
-    // Two levels, the fold order unchanged: each (lane, body) thread forms its
-    // body's two ratios; after a barrier one thread per lane folds the bodies in
-    // order with the same max2 (so NaN handling matches the sequential form).
-    __device__ void wnorm(int l0, int l1) {
-        const int nl = l1 - l0;
-        for (int t = threadIdx.x; t < nl * m; t += NTH) {
+        static_assert(LS * MAXM <= 64, "a line-search round's (lane, body) pairs fit one wave");
+        const int nl = l1 - l0, t = threadIdx.x;
+        double r0 = 0.0, r1 = 0.0;
+        if (t < nl * m) {
             const int lane = l0 + t / m, i = t % m;
+            gather_post_one(lane, i, lane == 0 && mass_of_candidate0);
             const double* r = RHS(lane, i);
             const double* sc = SCALE(lane, i);
-            double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
-            double* ra = RATIO(lane, i);
-            ra[0] = lin2 / sc[0];
-            ra[1] = ang2 / sc[1];
+            const double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
+            r0 = lin2 / sc[0];
+            r1 = ang2 / sc[1];
         }
-        __syncthreads();
-        for (int lane = l0 + (int)threadIdx.x; lane < l1; lane += NTH) {
+        if (t < 64) {
+            const int me = t < nl ? t : 0;
             double worst = 0.0;
             for (int i = 0; i < m; ++i) {
-                const double* ra = RATIO(lane, i);
-                worst = max2(worst, ra[0]);
-                worst = max2(worst, ra[1]);
+                const double a0 = __shfl(r0, me * m + i, 64), a1 = __shfl(r1, me * m + i, 64);
+                worst = max2(worst, a0);
+                worst = max2(worst, a1);
             }
-            s.wn[lane] = worst;
+            if (t < nl) s.wn[l0 + t] = worst;
         }
     }
     // ───── system matrix: A = 0·A + [i==j]·𝕀 + Σ terms (pf·∂W/∂pose + vf·∂W/∂vel) ─────
     __device__ void assemble() {
         double pf = -(half * half), vf = -half;
         double* Am = A();
-        for (int e = threadIdx.x; e < m * m * 6; e += NTH) {
-            const int b = e / 6, r = e - 6 * b, i = b / m, j = b - i * m;
+        // the non-zero blocks only (neighbour lists); the others stay as the launch
+        // zeroed them, which is what 0·A + 0 would write
+        for (int e = threadIdx.x; e < s.nbr_start[m] * 6; e += NTH) {
+            const int q = e / 6, r = e - 6 * q;
+            int i = 0;
+            while (s.nbr_start[i + 1] <= q) ++i;
+            const int j = s.nbr[q], b = i * m + j;
             const int q0 = s.term_start[b], q1 = s.term_start[b + 1];
             const double dg = (i == j) ? 1.0 : 0.0;
             double* dst = Am + (6 * i + r) * LD + 6 * j;
@@ -564,8 +564,6 @@ struct IslandSolver {
                 __syncthreads();
                 gather_post(0, 1);
                 __syncthreads();
-                wnorm(0, 1);
-                __syncthreads();
                 rnorm = uni(s.wn[0]);
             }
             if (rnorm < best) {
@@ -607,8 +605,6 @@ struct IslandSolver {
                 joints_plain(0, LS);
                 __syncthreads();
                 gather_post(0, LS, round == 0);
-                __syncthreads();
-                wnorm(0, LS);
                 __syncthreads();
                 for (int l = 0; l < LS; ++l)
                     if (uni(s.wn[l]) < rnorm) {

@@ -216,6 +216,43 @@ __host__ __device__ inline void axial_from_points(int kind, const double* params
     }
 }
 
+// The perpendicular damper's wrenches from its two world anchor points (the COMs):
+// soft length, unit direction, the perpendicular part of the relative velocity at a
+// (perpendicular_damper_warped.rs:107-162).
+template <class SP, class SV>
+__host__ __device__ inline void perp_from_points(const double* params, const SP& wap, const SP qap[3], const SP& wb,
+                                                 const SP qb[3], const SV vrel[6], double warp, SV wa_out[6],
+                                                 SV wb_out[6]) {
+    using RP = typename RealOf<SP>::type;
+    using RV = typename RealOf<SV>::type;
+    SP l[6];
+    join_points(wap, qap, wb, qb, l);
+    SP eps = SP(RP(params[1]));
+    SP dp = s_sqrt(line_norm2(l) + eps * eps);
+    SV wp = SV(RV(warp));
+    SV b_eff = SV(RV(params[0])) / (wp * wp);
+    SP dir[3];
+    line_direction(l, dir);
+    SP inv_d = cst<SP>(1.0) / dp;
+    SV n[3] = {promote<SV>(dir[0] * inv_d), promote<SV>(dir[1] * inv_d), promote<SV>(dir[2] * inv_d)};
+    SV wa = promote<SV>(wap), qa[3] = {promote<SV>(qap[0]), promote<SV>(qap[1]), promote<SV>(qap[2])};
+    SV v_rel[3];
+    velocity_at(vrel, wa, qa, v_rel);
+    SV v_dot_n = v_rel[0] * n[0] + (v_rel[1] * n[1] + (v_rel[2] * n[2] + cst<SV>(0.0)));
+    SV nb = -b_eff;
+    SV winv = cst<SV>(1.0) / wa;
+    SV tip[3];
+    for (int i = 0; i < 3; ++i) {
+        SV v_perp = v_rel[i] - n[i] * v_dot_n;
+        SV f = v_perp * nb;
+        tip[i] = qa[i] * winv + f;
+    }
+    SV l2[6];
+    join_points(cst<SV>(1.0), tip, wa, qa, l2);
+    wrench_from_line(l2, wa_out);
+    for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
+}
+
 // Wrenches [on a, on b] at world poses and world velocities (a pure world function).
 // `params` in the canonical order of each joint (joints/src/lib.rs:273-299).
 // The geometry (poses) is computed in SP and the velocity-dependent part in SV; with
@@ -237,36 +274,11 @@ __host__ __device__ inline void joint_wrench2(int kind, const double* params, co
         point_sandwich(pb_pose, lb, wb, qb);
         axial_from_points<SP, SV>(kind, params, wa, qa, wb, qb, vrel, dt, warp, wa_out, wb_out);
     } else if (kind == PERPENDICULAR_DAMPER_WARPED) {
-        SP z[3] = {cst<SP>(0.0), cst<SP>(0.0), cst<SP>(0.0)};
+        const RP z[3] = {RP(0.0), RP(0.0), RP(0.0)};
         SP wap, qap[3], wb, qb[3];
         point_sandwich(pa_pose, z, wap, qap);
         point_sandwich(pb_pose, z, wb, qb);
-        SP l[6];
-        join_points(wap, qap, wb, qb, l);
-        SP eps = SP(RP(params[1]));
-        SP dp = s_sqrt(line_norm2(l) + eps * eps);
-        SV wp = SV(RV(warp));
-        SV b_eff = SV(RV(params[0])) / (wp * wp);
-        SP dir[3];
-        line_direction(l, dir);
-        SP inv_d = cst<SP>(1.0) / dp;
-        SV n[3] = {promote<SV>(dir[0] * inv_d), promote<SV>(dir[1] * inv_d), promote<SV>(dir[2] * inv_d)};
-        SV wa = promote<SV>(wap), qa[3] = {promote<SV>(qap[0]), promote<SV>(qap[1]), promote<SV>(qap[2])};
-        SV v_rel[3];
-        velocity_at(vrel, wa, qa, v_rel);
-        SV v_dot_n = v_rel[0] * n[0] + (v_rel[1] * n[1] + (v_rel[2] * n[2] + cst<SV>(0.0)));
-        SV nb = -b_eff;
-        SV winv = cst<SV>(1.0) / wa;
-        SV tip[3];
-        for (int i = 0; i < 3; ++i) {
-            SV v_perp = v_rel[i] - n[i] * v_dot_n;
-            SV f = v_perp * nb;
-            tip[i] = qa[i] * winv + f;
-        }
-        SV l2[6];
-        join_points(cst<SV>(1.0), tip, wa, qa, l2);
-        wrench_from_line(l2, wa_out);
-        for (int i = 0; i < 6; ++i) wb_out[i] = -wa_out[i];
+        perp_from_points<SP, SV>(params, wap, qap, wb, qb, vrel, warp, wa_out, wb_out);
     } else {
         SV wp = SV(RV(warp));
         SV b_eff = SV(RV(params[0])) / (wp * wp);
@@ -318,9 +330,11 @@ __host__ __device__ __forceinline__ void joint_column(int kind, const double* pa
     const double* vels[2] = {vel_a, vel_b};
     for (int e = 0; e < 2; ++e)
         for (int k = 0; k < 6; ++k) vv[e][k] = D((T)vels[e][k], (ax >= 12 && (ax - 12) == 6 * e + k) ? T(1) : T(0));
-    if (ax < 12 && (kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED)) {
-        // pose axis of an axial joint: only the perturbed end's anchor carries a
-        // tangent; the other end's anchor is a plain sandwich (ends selected per lane)
+    const bool axial = kind == SIMPLE_SPRING_DAMPER || kind == CRITICALLY_DAMPED_WARPED;
+    if (ax < 12 && kind != TORSIONAL_DAMPER_WARPED) {
+        // pose axis: only the perturbed end's anchor carries a tangent; the other
+        // end's anchor is a plain sandwich (ends selected per lane). Anchors: the
+        // joint's attachment points (axial) or the COMs (perpendicular).
         const int e = ax / 6;
         T sb[8], ob[8], sa[3], oa[3];
         for (int k = 0; k < 8; ++k) {
@@ -328,8 +342,8 @@ __host__ __device__ __forceinline__ void joint_column(int kind, const double* pa
             ob[k] = (T)(e ? pose_a[k] : pose_b[k]);
         }
         for (int k = 0; k < 3; ++k) {
-            sa[k] = (T)params[e ? 3 + k : k];
-            oa[k] = (T)params[e ? k : 3 + k];
+            sa[k] = axial ? (T)params[e ? 3 + k : k] : T(0);
+            oa[k] = axial ? (T)params[e ? k : 3 + k] : T(0);
         }
         D ps[8];
         seed_pose(sb, ax % 6, ps);
@@ -346,17 +360,9 @@ __host__ __device__ __forceinline__ void joint_column(int kind, const double* pa
         }
         D vrel[6];
         for (int k = 0; k < 6; ++k) vrel[k] = vv[0][k] - vv[1][k];
-        axial_from_points<D, D>(kind, params, pa_w, pa_q, pb_w, pb_q, vrel, dt, warp, wa, wb);
-    } else if (ax < 12) {  // pose axis of another kind: tangents through the geometry
-        D pp[2][8];
-        for (int e = 0; e < 2; ++e) {
-            T base[8];
-            for (int k = 0; k < 8; ++k) base[k] = (T)poses[e][k];
-            if (ax / 6 == e) seed_pose(base, ax % 6, pp[e]);
-            else for (int k = 0; k < 8; ++k) pp[e][k] = D(base[k]);
-        }
-        joint_wrench2<D, D>(kind, params, pp[0], pp[1], vv[0], vv[1], dt, warp, wa, wb);
-    } else {  // velocity axis: the geometry carries no tangent
+        if (axial) axial_from_points<D, D>(kind, params, pa_w, pa_q, pb_w, pb_q, vrel, dt, warp, wa, wb);
+        else perp_from_points<D, D>(params, pa_w, pa_q, pb_w, pb_q, vrel, warp, wa, wb);
+    } else {  // velocity axis, or a torsional damper (no pose dependence): plain geometry
         T pz[2][8];
         for (int e = 0; e < 2; ++e)
             for (int k = 0; k < 8; ++k) pz[e][k] = (T)poses[e][k];
