@@ -511,7 +511,7 @@ int exg_arena_footprint(int n_bodies, int n_islands, int n_joints, long long inv
     const long long per_island = 3 * 8 + 5 * 4 + exg::ISLAND_STATS * 4 + 8 +      // origin, ranges, stats, offset,
                                  (long long)sizeof(exg::IslandTopo);             // topology record
     const long long per_joint = 10 * 8 + 3 * 4;                                   // params, kind, ends
-    const long long tiles = (n_bodies + 255) / 256;
+    const long long tiles = (n_bodies + 63) / 64;
     const long long chunks = (n_bodies + gravity_chunk - 1) / gravity_chunk;
     const long long grav = tiles * chunks * (exg::GRAV_PARTIAL_PER_ENTRY * 8 + (long long)sizeof(exg::GravEntry)) +
                            tiles * (long long)sizeof(exg::GravTile);

@@ -10,30 +10,26 @@
 // relative to the world frame, lever arm from the island origin, charge, core size).
 // Slots are body indices: bodies of one world are contiguous and in slot order.
 //
-// Work split: a table of entries, each one (tile of ≤ 256 targets, chunk of
-// sources of the same world). A workgroup takes one entry: each thread owns a
-// target and walks the chunk's sources, staged through LDS in blocks of 256 with
-// s² and 1/s of each source, and writes its partial sum to partial[entry][axis][lane]
-// (coalesced). A second table lists, per tile, its consecutive entries: the reduce
-// kernel adds them in ascending chunk order, scales by the target's charge / G and
-// forms the moment. A world's chunking depends only on its own body count, so its
-// forces are bit-identical whatever other worlds share the launch. About four
-// workgroups per CU fill all 256 CUs from a few thousand bodies up.
+// Work split: a table of entries, each one (tile of ≤ 64 targets, chunk of sources
+// of the same world). A workgroup of four waves takes one entry: every wave holds
+// the tile's 64 targets (one per lane) and walks a quarter of the chunk (its slice),
+// staging 64 sources at a time in its own LDS rows with s² and 1/s of each; source
+// blocks wholly below or above the tile's slots skip the per-pair slot test. Each
+// lane keeps two accumulators (even and odd sources, added at the end); the four
+// slice sums are added in slice order through LDS and written to
+// partial[entry][axis][lane]. A second table lists, per tile, its consecutive
+// entries: the reduce kernel adds them in ascending chunk order, scales by the
+// target's charge / G and forms the moment. A world's chunking depends only on its
+// own body count (and the CU count), so its forces are bit-identical whatever other
+// worlds share the launch. About eight workgroups per CU are aimed for.
 //
-// f64 pairs: one hardware reciprocal square root refined by two Newton steps
-// (y ← y + y·(1 − d²y²)/2) replaces the reference's sqrt and three divisions; the
-// result agrees with the correctly rounded form to a few ulp.
-// f32c pairs (mixed precision): sources and target are narrowed to f32 as they are
-// staged; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
-// component is accumulated as an unevaluated sum hi + lo — after every addition the
-// rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to lo.
-// The chunk sums are widened to f64 for the reduction, scaling and moment.
 #include <hip/hip_runtime.h>
 #include "island.h"
 
 namespace exg {
 
-constexpr int GRAV_T = 256;
+constexpr int GRAV_TILE = 64;     // targets per tile (one per lane)
+constexpr int GRAV_WAVES = 4;     // source slices per workgroup
 constexpr int GRAV_MIN_CHUNK = 64;
 
 __device__ inline double rsqrt_refined(double x) {
@@ -53,129 +49,211 @@ __device__ inline void two_sum_acc(float& hi, float& lo, float x) {
     lo = lo + e;
 }
 
-__global__ __launch_bounds__(GRAV_T) void gravity_partial_f64_kernel(const double* __restrict__ ch,
-                                                                     const GravEntry* __restrict__ entries,
-                                                                     double* __restrict__ partial) {
-    __shared__ double sx[GRAV_T], sy[GRAV_T], sz[GRAV_T], sg[GRAV_T], ss2[GRAV_T], sinv[GRAV_T];
+// Wave-local staging: a wave's LDS rows are written and read by that wave only.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The tile's slice [s0, s1) of a chunk, for one wave: block position MODE 0 = every
+// source below every target slot (the higher slot is the target), 1 = every source
+// above (the source), 2 = mixed (tested per pair, the pair itself skipped).
+template <int MODE>
+__device__ __forceinline__ void f64_block(const double2* sxy, const double2* szg, const double2* ssv, int cnt, int base,
+                                          int i, double xi, double yi, double zi, double si2, double sinv_i,
+                                          double f[2][3]) {
+    auto pair = [&](int q, double* acc) {
+        const int jj = base + q;
+        const double2 a = sxy[q], b = szg[q];
+        const double dx = a.x - xi, dy = a.y - yi, dz = b.x - zi;
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        double s2, inv_s;
+        if (MODE == 0) {
+            s2 = si2;
+            inv_s = sinv_i;
+        } else {
+            const double2 sv = ssv[q];
+            const bool h = MODE == 1 || jj > i;
+            s2 = h ? sv.x : si2;
+            inv_s = h ? sv.y : sinv_i;
+        }
+        const double inv_r = (d2 >= s2) ? rsqrt_refined(d2) : inv_s;
+        double w = b.y * (inv_r * inv_r * inv_r);
+        // the pair itself and coincident pairs (d < 1e-9) contribute nothing
+        w = ((MODE == 2 && jj == i) || d2 < 1e-18) ? 0.0 : w;
+        acc[0] = fma(w, dx, acc[0]);
+        acc[1] = fma(w, dy, acc[1]);
+        acc[2] = fma(w, dz, acc[2]);
+    };
+    int q = 0;
+    for (; q + 2 <= cnt; q += 2) {
+        pair(q, f[0]);
+        pair(q + 1, f[1]);
+    }
+    if (q < cnt) pair(q, f[0]);
+}
+
+template <int MODE>
+__device__ __forceinline__ void f32c_block(const float4* sp, const float2* sc, int cnt, int base, int i, float4 me,
+                                           float2 mc, float hi[2][3], float lo[2][3]) {
+    auto pair = [&](int q, float* h3, float* l3) {
+        const int jj = base + q;
+        const float4 o = sp[q];
+        const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        float2 cr = mc;
+        if (MODE != 0) {
+            const float2 sv = sc[q];
+            const bool h = MODE == 1 || jj > i;
+            cr = h ? sv : mc;
+        }
+        const float inv_r = (d2 >= cr.x) ? __builtin_amdgcn_rsqf(d2) : cr.y;
+        float w = o.w * (inv_r * inv_r * inv_r);
+        w = ((MODE == 2 && jj == i) || d2 < 1e-18f) ? 0.f : w;
+        two_sum_acc(h3[0], l3[0], w * dx);
+        two_sum_acc(h3[1], l3[1], w * dy);
+        two_sum_acc(h3[2], l3[2], w * dz);
+    };
+    int q = 0;
+    for (; q + 2 <= cnt; q += 2) {
+        pair(q, hi[0], lo[0]);
+        pair(q + 1, hi[1], lo[1]);
+    }
+    if (q < cnt) pair(q, hi[0], lo[0]);
+}
+
+// slice sums added in slice order; wave 0 writes the entry's partial
+__device__ inline void write_partial(double fx, double fy, double fz, bool live, double* __restrict__ partial) {
+    __shared__ double red[3][GRAV_WAVES - 1][GRAV_TILE];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (w > 0) {
+        red[0][w - 1][lane] = fx;
+        red[1][w - 1][lane] = fy;
+        red[2][w - 1][lane] = fz;
+    }
+    __syncthreads();
+    if (w == 0 && live) {
+        for (int k = 0; k < GRAV_WAVES - 1; ++k) {
+            fx += red[0][k][lane];
+            fy += red[1][k][lane];
+            fz += red[2][k][lane];
+        }
+        double* p = partial + (size_t)blockIdx.x * GRAV_PARTIAL_PER_ENTRY + lane;
+        p[0] = fx;
+        p[GRAV_TILE] = fy;
+        p[2 * GRAV_TILE] = fz;
+    }
+}
+
+__global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f64_kernel(const double* __restrict__ ch,
+                                                                              const GravEntry* __restrict__ entries,
+                                                                              double* __restrict__ partial) {
+    __shared__ double2 sxy[GRAV_WAVES][64], szg[GRAV_WAVES][64], ssv[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
-    const int i = en.tgt0 + threadIdx.x;
-    const bool live = (int)threadIdx.x < en.ntgt;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = en.tgt0 + lane;
+    const bool live = lane < en.ntgt;
     double xi = 0.0, yi = 0.0, zi = 0.0, si2 = 1.0, sinv_i = 1.0;
     if (live) {
         xi = ch[(size_t)i * 8 + 0];
         yi = ch[(size_t)i * 8 + 1];
         zi = ch[(size_t)i * 8 + 2];
-        double si = ch[(size_t)i * 8 + 7];
+        const double si = ch[(size_t)i * 8 + 7];
         si2 = si * si;
         sinv_i = 1.0 / si;
     }
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int base = en.src0; base < en.src1; base += GRAV_T) {
-        const int j = base + threadIdx.x;
-        if (j < en.src1) {
-            sx[threadIdx.x] = ch[(size_t)j * 8 + 0];
-            sy[threadIdx.x] = ch[(size_t)j * 8 + 1];
-            sz[threadIdx.x] = ch[(size_t)j * 8 + 2];
-            sg[threadIdx.x] = ch[(size_t)j * 8 + 6];
-            double sj = ch[(size_t)j * 8 + 7];
-            ss2[threadIdx.x] = sj * sj;
-            sinv[threadIdx.x] = 1.0 / sj;
+    const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
+    const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
+    const int t_lo = en.tgt0, t_hi = en.tgt0 + en.ntgt - 1;
+    double f[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    for (int base = s0; base < s1; base += 64) {
+        const int j = base + lane;
+        if (j < s1) {
+            const double* c = ch + (size_t)j * 8;
+            const double sj = c[7];
+            sxy[w][lane] = make_double2(c[0], c[1]);
+            szg[w][lane] = make_double2(c[2], c[6]);
+            ssv[w][lane] = make_double2(sj * sj, 1.0 / sj);
         }
-        __syncthreads();
-        const int cnt = min(GRAV_T, en.src1 - base);
-        if (live) {
-            for (int q = 0; q < cnt; ++q) {
-                const int jj = base + q;
-                double dx = sx[q] - xi, dy = sy[q] - yi, dz = sz[q] - zi;
-                double d2 = dx * dx + dy * dy + dz * dz;
-                bool hi_is_j = jj > i;
-                double s2 = hi_is_j ? ss2[q] : si2;
-                double inv_s = hi_is_j ? sinv[q] : sinv_i;
-                double inv_r = (d2 >= s2) ? rsqrt_refined(d2) : inv_s;
-                double w = sg[q] * (inv_r * inv_r * inv_r);
-                // the pair itself and coincident pairs (d < 1e-9) contribute nothing
-                w = (jj == i || d2 < 1e-18) ? 0.0 : w;
-                fx = fma(w, dx, fx);
-                fy = fma(w, dy, fy);
-                fz = fma(w, dz, fz);
-            }
-        }
-        __syncthreads();
+        wave_lds_sync();
+        const int cnt = min(64, s1 - base);
+        if (base + cnt - 1 < t_lo) f64_block<0>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
+        else if (base > t_hi) f64_block<1>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
+        else f64_block<2>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
+        wave_lds_sync();
     }
-    if (live) {
-        double* p = partial + (size_t)blockIdx.x * 3 * GRAV_T + threadIdx.x;
-        p[0] = fx;
-        p[GRAV_T] = fy;
-        p[2 * GRAV_T] = fz;
-    }
+    write_partial(f[0][0] + f[1][0], f[0][1] + f[1][1], f[0][2] + f[1][2], live, partial);
 }
 
-__global__ __launch_bounds__(GRAV_T) void gravity_partial_f32c_kernel(const double* __restrict__ ch,
-                                                                      const GravEntry* __restrict__ entries,
-                                                                      double* __restrict__ partial) {
-    __shared__ float4 sp[GRAV_T];
-    __shared__ float2 sc[GRAV_T];
+__global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f32c_kernel(const double* __restrict__ ch,
+                                                                               const GravEntry* __restrict__ entries,
+                                                                               double* __restrict__ partial) {
+    __shared__ float4 sp[GRAV_WAVES][64];
+    __shared__ float2 sc[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
-    const int i = en.tgt0 + threadIdx.x;
-    const bool live = (int)threadIdx.x < en.ntgt;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = en.tgt0 + lane;
+    const bool live = lane < en.ntgt;
     float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 mc = make_float2(1.f, 1.f);
     if (live) {
         const double* c = ch + (size_t)i * 8;
         me = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-        const float s = (float)c[7];
-        mc = make_float2(s * s, 1.0f / s);
+        const float sz = (float)c[7];
+        mc = make_float2(sz * sz, 1.0f / sz);
     }
-    float fx = 0.f, fy = 0.f, fz = 0.f, ex = 0.f, ey = 0.f, ez = 0.f;
-    for (int base = en.src0; base < en.src1; base += GRAV_T) {
-        const int j = base + threadIdx.x;
-        if (j < en.src1) {
+    const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
+    const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
+    const int t_lo = en.tgt0, t_hi = en.tgt0 + en.ntgt - 1;
+    float hi[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, lo[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    for (int base = s0; base < s1; base += 64) {
+        const int j = base + lane;
+        if (j < s1) {
             const double* c = ch + (size_t)j * 8;
-            sp[threadIdx.x] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-            const float s = (float)c[7];
-            sc[threadIdx.x] = make_float2(s * s, 1.0f / s);
+            sp[w][lane] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
+            const float sz = (float)c[7];
+            sc[w][lane] = make_float2(sz * sz, 1.0f / sz);
         }
-        __syncthreads();
-        const int cnt = min(GRAV_T, en.src1 - base);
-        if (live) {
-            for (int q = 0; q < cnt; ++q) {
-                const int jj = base + q;
-                const float4 o = sp[q];
-                const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                const bool hi_is_j = jj > i;
-                const float2 cr = hi_is_j ? sc[q] : mc;
-                const float inv_r = (d2 >= cr.x) ? __builtin_amdgcn_rsqf(d2) : cr.y;
-                float w = o.w * (inv_r * inv_r * inv_r);
-                w = (jj == i || d2 < 1e-18f) ? 0.f : w;
-                two_sum_acc(fx, ex, w * dx);
-                two_sum_acc(fy, ey, w * dy);
-                two_sum_acc(fz, ez, w * dz);
-            }
-        }
-        __syncthreads();
+        wave_lds_sync();
+        const int cnt = min(64, s1 - base);
+        if (base + cnt - 1 < t_lo) f32c_block<0>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
+        else if (base > t_hi) f32c_block<1>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
+        else f32c_block<2>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
+        wave_lds_sync();
     }
-    if (live) {
-        double* p = partial + (size_t)blockIdx.x * 3 * GRAV_T + threadIdx.x;
-        p[0] = (double)fx + (double)ex;
-        p[GRAV_T] = (double)fy + (double)ey;
-        p[2 * GRAV_T] = (double)fz + (double)ez;
-    }
+    double f[3];
+    for (int k = 0; k < 3; ++k) f[k] = ((double)hi[0][k] + (double)lo[0][k]) + ((double)hi[1][k] + (double)lo[1][k]);
+    write_partial(f[0], f[1], f[2], live, partial);
 }
 
-__global__ __launch_bounds__(GRAV_T) void gravity_reduce_kernel(const double* __restrict__ ch,
-                                                                const GravTile* __restrict__ tiles,
-                                                                const double* __restrict__ partial,
-                                                                double* __restrict__ ext) {
+__global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double* __restrict__ ch,
+                                                                   const GravTile* __restrict__ tiles,
+                                                                   const double* __restrict__ partial,
+                                                                   double* __restrict__ ext) {
     const GravTile tl = tiles[blockIdx.x];
-    if ((int)threadIdx.x >= tl.ntgt) return;
-    const int i = tl.tgt0 + threadIdx.x;
+    const int lane = threadIdx.x;
+    if (lane >= tl.ntgt) return;
+    const int i = tl.tgt0 + lane;
     double f[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < tl.ne; ++c)
-        for (int k = 0; k < 3; ++k) f[k] += partial[((size_t)(tl.e0 + c) * 3 + k) * GRAV_T + threadIdx.x];
+    int c = 0;
+    for (; c + 4 <= tl.ne; c += 4) {  // loads issued four chunks at a time, added in order
+        double v[4][3];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                v[u][k] = partial[(size_t)(tl.e0 + c + u) * GRAV_PARTIAL_PER_ENTRY + k * GRAV_TILE + lane];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f[k] += v[u][k];
+    }
+    for (; c < tl.ne; ++c)
+        for (int k = 0; k < 3; ++k) f[k] += partial[(size_t)(tl.e0 + c) * GRAV_PARTIAL_PER_ENTRY + k * GRAV_TILE + lane];
     const double a = ch[(size_t)i * 8 + 6] * tl.g_inv;
     for (int k = 0; k < 3; ++k) f[k] *= a;
-    double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
+    const double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
     double* o = ext + (size_t)i * 6;
     o[0] = f[0];
     o[1] = f[1];
@@ -187,16 +265,17 @@ __global__ __launch_bounds__(GRAV_T) void gravity_reduce_kernel(const double* __
 
 // Bodies of worlds without gravity get a zero wrench (a fill kernel, so a step
 // can be captured in a graph).
-__global__ __launch_bounds__(GRAV_T) void zero_wrench_kernel(double* __restrict__ ext, int b0, int n) {
-    const int t = blockIdx.x * GRAV_T + threadIdx.x;
+constexpr int ZERO_T = 256;
+__global__ __launch_bounds__(ZERO_T) void zero_wrench_kernel(double* __restrict__ ext, int b0, int n) {
+    const int t = blockIdx.x * ZERO_T + threadIdx.x;
     if (t < n * 6) ext[(size_t)b0 * 6 + t] = 0.0;
 }
 
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units) {
     if (n <= 0) return;
-    const int ntiles = (n + GRAV_T - 1) / GRAV_T;
-    const int want = 4 * (compute_units > 0 ? compute_units : 256);
+    const int ntiles = (n + GRAV_TILE - 1) / GRAV_TILE;
+    const int want = 8 * (compute_units > 0 ? compute_units : 256);
     int chunks = (want + ntiles - 1) / ntiles;
     const int max_chunks = (n + GRAV_MIN_CHUNK - 1) / GRAV_MIN_CHUNK;
     if (chunks > max_chunks) chunks = max_chunks;
@@ -207,13 +286,13 @@ void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& t
 void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                                int chunk_len) {
     if (n <= 0) return;
-    const int ntiles = (n + GRAV_T - 1) / GRAV_T;
+    const int ntiles = (n + GRAV_TILE - 1) / GRAV_TILE;
     if (chunk_len < 1) chunk_len = n;
     const int chunks = (n + chunk_len - 1) / chunk_len;
     for (int t = 0; t < ntiles; ++t) {
         GravTile tl{};
-        tl.tgt0 = b0 + t * GRAV_T;
-        tl.ntgt = std::min(GRAV_T, n - t * GRAV_T);
+        tl.tgt0 = b0 + t * GRAV_TILE;
+        tl.ntgt = std::min(GRAV_TILE, n - t * GRAV_TILE);
         tl.e0 = (int)entries.size();
         tl.ne = chunks;
         tl.g_inv = 1.0 / G;
@@ -233,17 +312,17 @@ void launch_gravity(const double* charges, const GravEntry* entries, int n_entri
                     int n_tiles, int precision, double* partial, double* ext, hipStream_t stream) {
     if (n_entries <= 0 || n_tiles <= 0) return;
     if (precision == 1)
-        hipLaunchKernelGGL(gravity_partial_f32c_kernel, dim3(n_entries), dim3(GRAV_T), 0, stream, charges, entries,
-                           partial);
+        hipLaunchKernelGGL(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), 0, stream, charges,
+                           entries, partial);
     else
-        hipLaunchKernelGGL(gravity_partial_f64_kernel, dim3(n_entries), dim3(GRAV_T), 0, stream, charges, entries,
-                           partial);
-    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_T), 0, stream, charges, tiles, partial, ext);
+        hipLaunchKernelGGL(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), 0, stream, charges,
+                           entries, partial);
+    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), 0, stream, charges, tiles, partial, ext);
 }
 
 void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(zero_wrench_kernel, dim3((n * 6 + GRAV_T - 1) / GRAV_T), dim3(GRAV_T), 0, stream, ext, b0, n);
+    hipLaunchKernelGGL(zero_wrench_kernel, dim3((n * 6 + ZERO_T - 1) / ZERO_T), dim3(ZERO_T), 0, stream, ext, b0, n);
 }
 
 }  // namespace exg

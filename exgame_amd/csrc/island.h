@@ -86,7 +86,7 @@ void launch_singles(const IslandArgs& a, hipStream_t stream);
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);
 
-// Pairwise gravity over a work table (gravity.hip): entries pair a tile of ≤ 256
+// Pairwise gravity over a work table (gravity.hip): entries pair a tile of ≤ 64
 // targets with a chunk of sources of the same world; tiles list their entries in
 // ascending chunk order. gravity_add_world appends one world's bodies [b0, b0+n)
 // with a chunking that depends only on n (and the CU count).
@@ -97,7 +97,7 @@ struct GravTile {
     int tgt0, ntgt, e0, ne;
     double g_inv;
 };
-constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * 256;   // doubles of partial sums per entry
+constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * 64;    // doubles of partial sums per entry
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units);
 // The same with a fixed chunk length (≥ 64 sources).
