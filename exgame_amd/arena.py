@@ -21,6 +21,8 @@ _SIGS = {
     "exg_arena_destroy": [C.c_void_p],
     "exg_arena_set_solver": [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int],
     "exg_arena_set_gravity_chunk": [C.c_void_p, C.c_int],
+    "exg_arena_set_graph": [C.c_void_p, C.c_int],
+    "exg_arena_autotune_gravity": [C.c_void_p, C.POINTER(C.c_int)],
     "exg_arena_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "exg_arena_admit": [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp, C.c_int, _ip, _ip, _ip, _dp,
                         C.c_double, C.c_int, _ip],
@@ -61,7 +63,7 @@ def footprint(scene, gravity_chunk: int = 1024) -> int:
 class Arena:
     def __init__(self, bodies: int, islands: int, joints: int, inverses: int, tol2: float = 1e-20,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
-                 gravity_chunk: int = 1024, jacobian_precision: str = "f64"):
+                 gravity_chunk: int = 1024, jacobian_precision: str = "f64", graph: bool = False):
         self.L = _bind()
         h = C.c_void_p()
         _lib.check(self.L.exg_arena_create(int(bodies), int(islands), int(joints), int(inverses), C.byref(h)))
@@ -72,7 +74,16 @@ class Arena:
         _lib.check(self.L.exg_arena_set_gravity_chunk(self.h, self.gravity_chunk))
         _lib.check(self.L.exg_arena_set_precision(self.h, PRECISIONS[gravity_precision],
                                                   PRECISIONS[jacobian_precision]))
+        _lib.check(self.L.exg_arena_set_graph(self.h, int(bool(graph))))
         self.shapes = {}
+
+    def autotune_gravity(self) -> int:
+        """Time the gravity pass of the live worlds over chunk lengths 128 … 8192 and
+        keep the fastest; returns it."""
+        out = C.c_int(0)
+        _lib.check(self.L.exg_arena_autotune_gravity(self.h, C.byref(out)))
+        self.gravity_chunk = out.value
+        return out.value
 
     def admit(self, scene, G: float = 1.0, gravity: bool = True):
         """World id, or None when the arena has no room for it now."""

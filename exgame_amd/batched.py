@@ -7,6 +7,10 @@ h = 1e-2, Newton tolerance 1e-10. Prints one JSON line: body-steps/s over a time
 rollout, the device time per step, and the precision used.
 
     python -m exgame_amd.batched [--worlds 512] [--steps 20] [--precision f64|f32c]
+                                 [--jacobian f64|f32c] [--chunk N | --autotune] [--graph]
+
+`--autotune` times the gravity pass over chunk lengths 128 … 8192 on the admitted
+worlds and keeps the fastest; `--graph` replays two steps at a time from a HIP graph.
 """
 from __future__ import annotations
 
@@ -36,15 +40,21 @@ def main() -> int:
     ap.add_argument("--precision", default="f64", choices=["f64", "f32c"], help="gravity arithmetic")
     ap.add_argument("--jacobian", default=None, choices=["f64", "f32c"], help="Jacobian arithmetic (default: as gravity)")
     ap.add_argument("--chunk", type=int, default=1024)
+    ap.add_argument("--autotune", action="store_true", help="pick the gravity chunk length by timing")
+    ap.add_argument("--graph", action="store_true", help="replay steps from a HIP graph")
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     worlds = batch_worlds(args.worlds, args.seed)
     B = sum(w.n_bodies for w in worlds)
     ar = Arena(bodies=B, islands=sum(w.n_islands for w in worlds), joints=sum(w.n_joints for w in worlds) + 1,
                inverses=sum(inverse_doubles(w) for w in worlds), tol2=1e-20, gravity_precision=args.precision,
-               gravity_chunk=args.chunk, jacobian_precision=args.jacobian or args.precision)
+               gravity_chunk=args.chunk, jacobian_precision=args.jacobian or args.precision, graph=args.graph)
     ids = [ar.admit(w) for w in worlds]
     assert all(i is not None for i in ids)
+    if args.autotune:
+        ar.step(1, 1e-2, 1.0)     # charges past the bootstrap frame
+        ar.sync()
+        ar.autotune_gravity()
     ar.step(args.warmup, 1e-2, 1.0)
     ar.sync()
     t0 = time.perf_counter()
@@ -55,7 +65,7 @@ def main() -> int:
             "body_steps_per_s": B * args.steps / el, "ms_per_step": el * 1e3 / args.steps,
             "device_ms_per_step": ar.last_ms() / args.steps, "precision": args.precision,
             "jacobian": args.jacobian or args.precision,
-            "gravity_chunk": args.chunk,
+            "gravity_chunk": ar.gravity_chunk, "autotuned": bool(args.autotune), "graph": bool(args.graph),
             "islands": sum(w.n_islands for w in worlds), "pairs_per_step": int(sum(w.n_bodies ** 2 for w in worlds))}
     print(json.dumps(line), flush=True)
     return 0

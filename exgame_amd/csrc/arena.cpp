@@ -110,6 +110,13 @@ struct exg_arena {
     bool dirty = true;
     int front = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    bool use_graph = false;
+    int settings_epoch = 0;   // bumped by anything baked into captured kernel arguments
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+        double dt = 0.0, warp = 0.0;
+        int settings = -1;
+    } graphs[2];              // by the charge buffer the two-step graph starts from
 
     void release() {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
@@ -124,6 +131,8 @@ struct exg_arena {
         for (auto& c : class_ids) c.release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -139,10 +148,10 @@ struct exg_arena {
                 exg::gravity_add_world_chunked(en, tl, (int)w.b0, (int)w.nb, w.G, grav_chunk);
             } else {
                 // tiles without entries: the reduce writes a zero wrench
-                for (long long t = 0; t < w.nb; t += 256) {
+                for (long long t = 0; t < w.nb; t += exg::GRAV_TILE) {
                     exg::GravTile z{};
                     z.tgt0 = (int)(w.b0 + t);
-                    z.ntgt = (int)std::min<long long>(256, w.nb - t);
+                    z.ntgt = (int)std::min<long long>(exg::GRAV_TILE, w.nb - t);
                     z.e0 = 0;
                     z.ne = 0;
                     z.g_inv = 1.0;
@@ -175,6 +184,7 @@ struct exg_arena {
             A_TRY(class_ids[c].upload(ids[c].data(), ids[c].size()));
         }
         dirty = false;
+        settings_epoch++;
         return EXG_OK;
     }
 };
@@ -261,6 +271,7 @@ int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, in
     r->ns_seed = ns_seed;
     r->ns_engine = ns_engine;
     r->grav_prec = gravity_precision;
+    r->settings_epoch++;
     return EXG_OK;
 }
 
@@ -270,6 +281,7 @@ int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian) {
     if (!known(gravity) || !known(jacobian)) return afail(EXG_ERR_INVALID, "exg_arena_set_precision: unknown precision");
     r->grav_prec = gravity;
     r->jac_prec = jacobian;
+    r->settings_epoch++;
     return EXG_OK;
 }
 
@@ -398,6 +410,87 @@ int exg_arena_retire(exg_arena* r, int world_id) {
     return EXG_OK;
 }
 
+// One step of every live world, enqueued on the arena's stream.
+static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
+    double* front = r->charges[r->front].p;
+    double* back = r->charges[1 - r->front].p;
+    exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
+                        r->partial.p, r->ext.p, r->stream);
+    A_TRY(hipGetLastError());
+    exg::IslandArgs a{};
+    a.pose = r->pose.p;
+    a.mom = r->mom.p;
+    a.mass = r->mass.p;
+    a.tens = r->tens.p;
+    a.size = r->size.p;
+    a.ext = r->ext.p;
+    a.island_start = r->island_start.p;
+    a.island_end = r->island_end.p;
+    a.island_jstart = r->island_jstart.p;
+    a.island_jend = r->island_jend.p;
+    a.origin = r->origin.p;
+    a.jkind = r->jkind.p;
+    a.ja = r->ja.p;
+    a.jb = r->jb.p;
+    a.jparams = r->jparams.p;
+    a.xacc = r->xacc.p;
+    a.xacc_off = r->xacc_off.p;
+    a.topo = r->topo.p;
+    a.seeded = r->seeded.p;
+    a.charges_out = back;
+    a.stats = r->stats.p;
+    a.dt = dt;
+    a.warp = warp;
+    a.tol2 = r->tol2;
+    a.G = 1.0;
+    a.G_of_island = r->island_G.p;  // each world publishes G·m with its own G
+    a.ns_warm = r->ns_warm;
+    a.ns_seed = r->ns_seed;
+    a.ns_engine = r->ns_engine;
+    a.jac_f32 = r->jac_prec == EXG_PREC_F32_COMPENSATED;
+    for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
+        if (!r->class_n[c]) continue;
+        a.island_ids = r->class_ids[c].p;
+        a.n_islands = r->class_n[c];
+        exg::launch_island_class(a, c, r->stream);
+    }
+    A_TRY(hipGetLastError());
+    r->front = 1 - r->front;
+    return EXG_OK;
+}
+
+// Graph of two consecutive steps from charge buffer `front` (one round trip of the
+// double buffer), re-captured when dt, warp, a setting or the set of worlds changed.
+static int arena_graph(exg_arena* r, double dt, double warp, hipGraphExec_t* out) {
+    auto& g = r->graphs[r->front];
+    if (g.exec && g.dt == dt && g.warp == warp && g.settings == r->settings_epoch) {
+        *out = g.exec;
+        return EXG_OK;
+    }
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+    const int front0 = r->front;
+    A_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_arena_step(r, dt, warp);
+    if (rc == EXG_OK) rc = enqueue_arena_step(r, dt, warp);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(r->stream, &graph);
+    r->front = front0;
+    if (rc != EXG_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (e != hipSuccess) return afail(EXG_ERR_BACKEND, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return afail(EXG_ERR_BACKEND, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    g.dt = dt;
+    g.warp = warp;
+    g.settings = r->settings_epoch;
+    *out = g.exec;
+    return EXG_OK;
+}
+
 int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_step: null arena");
     if (n_steps < 0) return afail(EXG_ERR_INVALID, "exg_arena_step: negative step count");
@@ -407,53 +500,60 @@ int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp) {
     for (auto& w : r->worlds) any |= w.live;
     if (!any || n_steps == 0) return EXG_OK;
     A_TRY(hipEventRecord(r->ev[0], r->stream));
-    for (int s = 0; s < n_steps; ++s) {
+    int s = 0;
+    if (r->use_graph && n_steps >= 2) {
+        hipGraphExec_t exec = nullptr;
+        if (int rc = arena_graph(r, dt, warp, &exec)) return rc;
+        for (; s + 1 < n_steps; s += 2) A_TRY(hipGraphLaunch(exec, r->stream));
+    }
+    for (; s < n_steps; ++s)
+        if (int rc = enqueue_arena_step(r, dt, warp)) return rc;
+    A_TRY(hipEventRecord(r->ev[1], r->stream));
+    return EXG_OK;
+}
+
+int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_autotune_gravity: null arena");
+    bool any = false;
+    for (auto& w : r->worlds) any |= w.live && w.gravity_on;
+    if (!any) {
+        if (chosen_chunk) *chosen_chunk = r->grav_chunk;
+        return EXG_OK;
+    }
+    // each candidate: its work table, one warm pass, then the mean of five timed passes
+    // of the gravity kernels on the current charges (the next step recomputes ext)
+    static const int candidates[] = {128, 256, 512, 1024, 2048, 4096, 8192};
+    int best = r->grav_chunk;
+    float best_ms = -1.0f;
+    for (int c : candidates) {
+        r->grav_chunk = c;
+        if (int rc = r->rebuild()) return rc;
         double* front = r->charges[r->front].p;
-        double* back = r->charges[1 - r->front].p;
         exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
                             r->partial.p, r->ext.p, r->stream);
+        A_TRY(hipEventRecord(r->ev[0], r->stream));
+        for (int k = 0; k < 5; ++k)
+            exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
+                                r->partial.p, r->ext.p, r->stream);
+        A_TRY(hipEventRecord(r->ev[1], r->stream));
         A_TRY(hipGetLastError());
-        exg::IslandArgs a{};
-        a.pose = r->pose.p;
-        a.mom = r->mom.p;
-        a.mass = r->mass.p;
-        a.tens = r->tens.p;
-        a.size = r->size.p;
-        a.ext = r->ext.p;
-        a.island_start = r->island_start.p;
-        a.island_end = r->island_end.p;
-        a.island_jstart = r->island_jstart.p;
-        a.island_jend = r->island_jend.p;
-        a.origin = r->origin.p;
-        a.jkind = r->jkind.p;
-        a.ja = r->ja.p;
-        a.jb = r->jb.p;
-        a.jparams = r->jparams.p;
-        a.xacc = r->xacc.p;
-        a.xacc_off = r->xacc_off.p;
-        a.topo = r->topo.p;
-        a.seeded = r->seeded.p;
-        a.charges_out = back;
-        a.stats = r->stats.p;
-        a.dt = dt;
-        a.warp = warp;
-        a.tol2 = r->tol2;
-        a.G = 1.0;
-        a.G_of_island = r->island_G.p;  // each world publishes G·m with its own G
-        a.ns_warm = r->ns_warm;
-        a.ns_seed = r->ns_seed;
-        a.ns_engine = r->ns_engine;
-        a.jac_f32 = r->jac_prec == EXG_PREC_F32_COMPENSATED;
-        for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
-            if (!r->class_n[c]) continue;
-            a.island_ids = r->class_ids[c].p;
-            a.n_islands = r->class_n[c];
-            exg::launch_island_class(a, c, r->stream);
+        A_TRY(hipEventSynchronize(r->ev[1]));
+        float ms = 0.0f;
+        A_TRY(hipEventElapsedTime(&ms, r->ev[0], r->ev[1]));
+        if (best_ms < 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
         }
-        A_TRY(hipGetLastError());
-        r->front = 1 - r->front;
     }
-    A_TRY(hipEventRecord(r->ev[1], r->stream));
+    r->grav_chunk = best;
+    if (int rc = r->rebuild()) return rc;
+    if (chosen_chunk) *chosen_chunk = best;
+    return EXG_OK;
+}
+
+int exg_arena_set_graph(exg_arena* r, int enable) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_graph: null arena");
+    r->use_graph = enable != 0;
     return EXG_OK;
 }
 

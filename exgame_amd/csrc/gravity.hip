@@ -28,7 +28,7 @@
 
 namespace exg {
 
-constexpr int GRAV_TILE = 64;     // targets per tile (one per lane)
+static_assert(GRAV_TILE == 64, "one target per lane of a wave");
 constexpr int GRAV_WAVES = 4;     // source slices per workgroup
 constexpr int GRAV_MIN_CHUNK = 64;
 
@@ -235,6 +235,11 @@ __global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double*
     const int lane = threadIdx.x;
     if (lane >= tl.ntgt) return;
     const int i = tl.tgt0 + lane;
+    double* o = ext + (size_t)i * 6;
+    if (tl.ne == 0) {  // a world without gravity
+        for (int k = 0; k < 6; ++k) o[k] = 0.0;
+        return;
+    }
     double f[3] = {0.0, 0.0, 0.0};
     int c = 0;
     for (; c + 4 <= tl.ne; c += 4) {  // loads issued four chunks at a time, added in order
@@ -254,7 +259,6 @@ __global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double*
     const double a = ch[(size_t)i * 8 + 6] * tl.g_inv;
     for (int k = 0; k < 3; ++k) f[k] *= a;
     const double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
-    double* o = ext + (size_t)i * 6;
     o[0] = f[0];
     o[1] = f[1];
     o[2] = f[2];
@@ -310,7 +314,14 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
 
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
                     int n_tiles, int precision, double* partial, double* ext, hipStream_t stream) {
-    if (n_entries <= 0 || n_tiles <= 0) return;
+    if (n_tiles <= 0) return;
+    // tiles without entries (worlds without gravity) still get their zero wrench from
+    // the reduce, also when no world of the launch has gravity
+    if (n_entries <= 0) {
+        hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), 0, stream, charges, tiles, partial,
+                           ext);
+        return;
+    }
     if (precision == 1)
         hipLaunchKernelGGL(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), 0, stream, charges,
                            entries, partial);

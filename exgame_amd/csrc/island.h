@@ -97,7 +97,8 @@ struct GravTile {
     int tgt0, ntgt, e0, ne;
     double g_inv;
 };
-constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * 64;    // doubles of partial sums per entry
+constexpr int GRAV_TILE = 64;                     // targets per tile
+constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * GRAV_TILE;   // doubles of partial sums per entry
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units);
 // The same with a fixed chunk length (≥ 64 sources).

@@ -231,6 +231,15 @@ int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, in
 int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian);
 /* Gravity work split: sources per chunk (≥ 64), the same for every world. */
 int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len);
+/* Replay steps from a captured HIP graph of two consecutive steps of every live
+ * world (re-captured when dt, warp, a setting or the set of worlds changes). No
+ * counterpart in the reference (its Driver records command buffers per flush). */
+int exg_arena_set_graph(exg_arena* r, int enable);
+/* Time the gravity pass of the live worlds for chunk lengths 128 … 8192 and keep the
+ * fastest (written to *chosen_chunk when not null). The chunking sets the order of
+ * each world's force sums, so results are bit-identical across arenas only at equal
+ * chunk lengths. No counterpart in the reference. */
+int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk);
 /* Admit a world (arrays as for exg_world_create); EXG_ERR_FULL when no range fits. */
 int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double* momentum, const double* mass,
                     const double* inertia, const double* size, int n_islands, const int* island_start,

@@ -105,6 +105,35 @@ def test_worlds_without_gravity_and_with_their_own_G():
     assert inverse_doubles(scene) == 64 * 36
 
 
+def test_gravity_autotune_keeps_parity():
+    scene = scenes.mixed_world(600, seed=8)
+    ar = big_arena(tol2=1e-20)
+    w = ar.admit(scene)
+    ar.step(1, 1e-2, 1.0)
+    chunk = ar.autotune_gravity()
+    assert chunk in (128, 256, 512, 1024, 2048, 4096, 8192)
+    ar.step(2, 1e-2, 1.0)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    orc.step(3, 1e-2, 1.0)
+    assert rel(ar.state(w)["position"], orc.state()["position"]) < 1e-9
+
+
+def test_world_without_gravity_in_a_range_that_had_gravity():
+    # every body of a gravity-off world gets a zero wrench, also beyond the first tile
+    # of targets and where a retired gravitating world left its wrenches
+    scene = scenes.plain_gravity(n=200, seed=5)
+    ar = big_arena(tol2=1e-20)
+    w_on = ar.admit(scene)
+    ar.step(3, 1e-3, 1.0)
+    ar.retire(w_on)
+    w_off = ar.admit(scene, gravity=False)
+    ar.step(3, 1e-3, 1.0)
+    orc_off = pyoracle.OracleWorld(scene, tol2=1e-20, gravity=False)
+    orc_off.step(3, 1e-3, 1.0)
+    assert rel(ar.state(w_off)["position"], orc_off.state()["position"]) < 1e-12
+    assert rel(ar.state(w_off)["momentum"], orc_off.state()["momentum"]) < 1e-12
+
+
 def test_served_requests_equal_their_alone_runs():
     from exgame_amd.arena import footprint
     from exgame_amd.serve import Server, arena_for, synthetic_requests

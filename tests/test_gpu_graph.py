@@ -27,3 +27,30 @@ def test_graph_replays_equal_eager_steps(precision):
     np.testing.assert_array_equal(eager.stats(), graph.stats())
     t = graph.last_timing()
     assert t["gravity_ms"] == -1.0 and t["island_ms"] > 0.0
+
+
+def test_arena_graph_replays_equal_eager_steps_across_admit_and_retire():
+    # worlds admitted and retired between replays (the graph is re-captured for the new
+    # set of worlds); every state must equal the eager arena's bit for bit
+    from exgame_amd.arena import Arena
+    worlds = [scenes.mixed_world(n, seed=20 + n, rigs=True) for n in (160, 240, 96)]
+    arenas = [Arena(bodies=4096, islands=2048, joints=4096, inverses=1 << 20, tol2=1e-20, graph=g)
+              for g in (False, True)]
+    ids = [[ar.admit(worlds[0]), ar.admit(worlds[1])] for ar in arenas]
+    for n, dt in [(4, 1e-2), (3, 1e-2), (2, 5e-3)]:
+        for ar in arenas:
+            ar.step(n, dt, 1.0)
+            ar.sync()
+        for w in range(len(ids[0])):
+            a, b = arenas[0].state(ids[0][w]), arenas[1].state(ids[1][w])
+            for k in ("pose", "momentum", "origin"):
+                assert np.array_equal(a[k], b[k]), (n, dt, w, k)
+    for ar, wid in zip(arenas, ids):
+        ar.retire(wid[0])
+        wid[0] = ar.admit(worlds[2])
+        ar.step(4, 1e-2, 1.0)
+        ar.sync()
+    for w in range(2):
+        a, b = arenas[0].state(ids[0][w]), arenas[1].state(ids[1][w])
+        for k in ("pose", "momentum", "origin"):
+            assert np.array_equal(a[k], b[k]), (w, k)
