@@ -13,7 +13,7 @@
 // Work split: a table of entries, each one (tile of ≤ 64 targets, chunk of sources
 // of the same world). A workgroup of four waves takes one entry: every wave holds
 // the tile's 64 targets (one per lane) and walks a quarter of the chunk (its slice),
-// staging 64 sources at a time in its own LDS rows with s² and 1/s of each; source
+// staging 64 sources at a time in its own LDS rows with 1/s of each; source
 // blocks wholly below or above the tile's slots skip the per-pair slot test. Each
 // lane keeps two accumulators (even and odd sources, added at the end); the four
 // slice sums are added in slice order through LDS and written to
@@ -21,8 +21,16 @@
 // entries: the reduce kernel adds them in ascending chunk order, scales by the
 // target's charge / G and forms the moment. A world's chunking depends only on its
 // own body count (and the CU count), so its forces are bit-identical whatever other
-// worlds share the launch. About eight workgroups per CU are aimed for.
+// worlds share the launch.
 //
+// f64 pairs: one hardware reciprocal square root refined by a third-order step
+// replaces the reference's sqrt and three divisions; the result is within 2 ulp of
+// the correctly rounded form.
+// f32c pairs (mixed precision): sources and target are narrowed to f32 as they are
+// staged; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
+// component is accumulated as an unevaluated sum hi + lo — after every addition the
+// rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to lo.
+// The slice sums are widened to f64 for the reductions, scaling and moment.
 #include <hip/hip_runtime.h>
 #include "island.h"
 
@@ -32,13 +40,14 @@ static_assert(GRAV_TILE == 64, "one target per lane of a wave");
 constexpr int GRAV_WAVES = 4;     // source slices per workgroup
 constexpr int GRAV_MIN_CHUNK = 64;
 
+// The hardware estimate (≈ 2^-24 relative) refined by one third-order step,
+// y = y0 + y0·e·(1/2 + 3/8·e) with e = 1 − x·y0²: within 2 ulp of 1/sqrt(x) over
+// 1e-40 … 2e30 (the bound of two Newton steps, in five operations instead of eight).
 __device__ inline double rsqrt_refined(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    double e = fma(-(x * y), y, 1.0);
-    y = fma(y * 0.5, e, y);
-    e = fma(-(x * y), y, 1.0);
-    y = fma(y * 0.5, e, y);
-    return y;
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y0), y0, 1.0);
+    const double c = fma(0.375, e, 0.5);
+    return fma(y0 * e, c, y0);
 }
 
 __device__ inline void two_sum_acc(float& hi, float& lo, float x) {
@@ -59,26 +68,20 @@ __device__ inline void wave_lds_sync() {
 // The tile's slice [s0, s1) of a chunk, for one wave: block position MODE 0 = every
 // source below every target slot (the higher slot is the target), 1 = every source
 // above (the source), 2 = mixed (tested per pair, the pair itself skipped).
+// The core law r = max(d, s) enters as 1/r = min(1/d, 1/s): branch-free, and for a
+// coincident pair (d = 0, whose refined 1/d is not a number) min returns 1/s.
 template <int MODE>
-__device__ __forceinline__ void f64_block(const double2* sxy, const double2* szg, const double2* ssv, int cnt, int base,
-                                          int i, double xi, double yi, double zi, double si2, double sinv_i,
-                                          double f[2][3]) {
+__device__ __forceinline__ void f64_block(const double2* sxy, const double2* szg, const double* sinv, int cnt, int base,
+                                          int i, double xi, double yi, double zi, double sinv_i, double f[2][3]) {
     auto pair = [&](int q, double* acc) {
         const int jj = base + q;
         const double2 a = sxy[q], b = szg[q];
         const double dx = a.x - xi, dy = a.y - yi, dz = b.x - zi;
         const double d2 = dx * dx + dy * dy + dz * dz;
-        double s2, inv_s;
-        if (MODE == 0) {
-            s2 = si2;
-            inv_s = sinv_i;
-        } else {
-            const double2 sv = ssv[q];
-            const bool h = MODE == 1 || jj > i;
-            s2 = h ? sv.x : si2;
-            inv_s = h ? sv.y : sinv_i;
-        }
-        const double inv_r = (d2 >= s2) ? rsqrt_refined(d2) : inv_s;
+        double inv_s = sinv_i;
+        if (MODE == 1) inv_s = sinv[q];
+        if (MODE == 2) inv_s = jj > i ? sinv[q] : sinv_i;
+        const double inv_r = fmin(rsqrt_refined(d2), inv_s);
         double w = b.y * (inv_r * inv_r * inv_r);
         // the pair itself and coincident pairs (d < 1e-9) contribute nothing
         w = ((MODE == 2 && jj == i) || d2 < 1e-18) ? 0.0 : w;
@@ -95,20 +98,17 @@ __device__ __forceinline__ void f64_block(const double2* sxy, const double2* szg
 }
 
 template <int MODE>
-__device__ __forceinline__ void f32c_block(const float4* sp, const float2* sc, int cnt, int base, int i, float4 me,
-                                           float2 mc, float hi[2][3], float lo[2][3]) {
+__device__ __forceinline__ void f32c_block(const float4* sp, const float* sc, int cnt, int base, int i, float4 me,
+                                           float minv, float hi[2][3], float lo[2][3]) {
     auto pair = [&](int q, float* h3, float* l3) {
         const int jj = base + q;
         const float4 o = sp[q];
         const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
         const float d2 = dx * dx + dy * dy + dz * dz;
-        float2 cr = mc;
-        if (MODE != 0) {
-            const float2 sv = sc[q];
-            const bool h = MODE == 1 || jj > i;
-            cr = h ? sv : mc;
-        }
-        const float inv_r = (d2 >= cr.x) ? __builtin_amdgcn_rsqf(d2) : cr.y;
+        float inv_s = minv;
+        if (MODE == 1) inv_s = sc[q];
+        if (MODE == 2) inv_s = jj > i ? sc[q] : minv;
+        const float inv_r = fminf(__builtin_amdgcn_rsqf(d2), inv_s);
         float w = o.w * (inv_r * inv_r * inv_r);
         w = ((MODE == 2 && jj == i) || d2 < 1e-18f) ? 0.f : w;
         two_sum_acc(h3[0], l3[0], w * dx);
@@ -149,19 +149,18 @@ __device__ inline void write_partial(double fx, double fy, double fz, bool live,
 __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f64_kernel(const double* __restrict__ ch,
                                                                               const GravEntry* __restrict__ entries,
                                                                               double* __restrict__ partial) {
-    __shared__ double2 sxy[GRAV_WAVES][64], szg[GRAV_WAVES][64], ssv[GRAV_WAVES][64];
+    __shared__ double2 sxy[GRAV_WAVES][64], szg[GRAV_WAVES][64];
+    __shared__ double sinv[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = en.tgt0 + lane;
     const bool live = lane < en.ntgt;
-    double xi = 0.0, yi = 0.0, zi = 0.0, si2 = 1.0, sinv_i = 1.0;
+    double xi = 0.0, yi = 0.0, zi = 0.0, sinv_i = 1.0;
     if (live) {
         xi = ch[(size_t)i * 8 + 0];
         yi = ch[(size_t)i * 8 + 1];
         zi = ch[(size_t)i * 8 + 2];
-        const double si = ch[(size_t)i * 8 + 7];
-        si2 = si * si;
-        sinv_i = 1.0 / si;
+        sinv_i = 1.0 / ch[(size_t)i * 8 + 7];
     }
     const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
     const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
@@ -171,16 +170,15 @@ __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f64_kernel(co
         const int j = base + lane;
         if (j < s1) {
             const double* c = ch + (size_t)j * 8;
-            const double sj = c[7];
             sxy[w][lane] = make_double2(c[0], c[1]);
             szg[w][lane] = make_double2(c[2], c[6]);
-            ssv[w][lane] = make_double2(sj * sj, 1.0 / sj);
+            sinv[w][lane] = 1.0 / c[7];
         }
         wave_lds_sync();
         const int cnt = min(64, s1 - base);
-        if (base + cnt - 1 < t_lo) f64_block<0>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
-        else if (base > t_hi) f64_block<1>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
-        else f64_block<2>(sxy[w], szg[w], ssv[w], cnt, base, i, xi, yi, zi, si2, sinv_i, f);
+        if (base + cnt - 1 < t_lo) f64_block<0>(sxy[w], szg[w], sinv[w], cnt, base, i, xi, yi, zi, sinv_i, f);
+        else if (base > t_hi) f64_block<1>(sxy[w], szg[w], sinv[w], cnt, base, i, xi, yi, zi, sinv_i, f);
+        else f64_block<2>(sxy[w], szg[w], sinv[w], cnt, base, i, xi, yi, zi, sinv_i, f);
         wave_lds_sync();
     }
     write_partial(f[0][0] + f[1][0], f[0][1] + f[1][1], f[0][2] + f[1][2], live, partial);
@@ -190,18 +188,17 @@ __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f32c_kernel(c
                                                                                const GravEntry* __restrict__ entries,
                                                                                double* __restrict__ partial) {
     __shared__ float4 sp[GRAV_WAVES][64];
-    __shared__ float2 sc[GRAV_WAVES][64];
+    __shared__ float sc[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = en.tgt0 + lane;
     const bool live = lane < en.ntgt;
     float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
-    float2 mc = make_float2(1.f, 1.f);
+    float minv = 1.f;
     if (live) {
         const double* c = ch + (size_t)i * 8;
         me = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-        const float sz = (float)c[7];
-        mc = make_float2(sz * sz, 1.0f / sz);
+        minv = 1.0f / (float)c[7];
     }
     const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
     const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
@@ -212,14 +209,13 @@ __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f32c_kernel(c
         if (j < s1) {
             const double* c = ch + (size_t)j * 8;
             sp[w][lane] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-            const float sz = (float)c[7];
-            sc[w][lane] = make_float2(sz * sz, 1.0f / sz);
+            sc[w][lane] = 1.0f / (float)c[7];
         }
         wave_lds_sync();
         const int cnt = min(64, s1 - base);
-        if (base + cnt - 1 < t_lo) f32c_block<0>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
-        else if (base > t_hi) f32c_block<1>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
-        else f32c_block<2>(sp[w], sc[w], cnt, base, i, me, mc, hi, lo);
+        if (base + cnt - 1 < t_lo) f32c_block<0>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
+        else if (base > t_hi) f32c_block<1>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
+        else f32c_block<2>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
         wave_lds_sync();
     }
     double f[3];
@@ -278,9 +274,14 @@ __global__ __launch_bounds__(ZERO_T) void zero_wrench_kernel(double* __restrict_
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units) {
     if (n <= 0) return;
+    // enough workgroups to fill the chip (4 per CU), chunks of at most 1,024 sources
+    // (long chunks leave the later waves idle), at most 64 workgroups per CU (the
+    // partial sums stay small), at least 64 sources per chunk
     const int ntiles = (n + GRAV_TILE - 1) / GRAV_TILE;
-    const int want = 8 * (compute_units > 0 ? compute_units : 256);
-    int chunks = (want + ntiles - 1) / ntiles;
+    const int cus = compute_units > 0 ? compute_units : 256;
+    int chunks = (4 * cus + ntiles - 1) / ntiles;
+    chunks = std::max(chunks, (n + 1023) / 1024);
+    chunks = std::min(chunks, std::max(1, 64 * cus / ntiles));
     const int max_chunks = (n + GRAV_MIN_CHUNK - 1) / GRAV_MIN_CHUNK;
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
