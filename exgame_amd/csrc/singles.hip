@@ -53,21 +53,32 @@ __device__ inline double reduce_block(const double (*b)[ST], int t, double diag)
 // out = 0·out + a·(sign·b + diag·2I)  (fn_gemm_term with one neighbour block),
 // a row at a time. `out` may be `a` (not `b`): row r of the result needs only row r
 // of `a`, which is read before the row is stored.
+// The right operand (sign·b + diag·2I) is formed once in registers.
 __device__ inline void gemm_block(const double (*a)[ST], const double (*b)[ST], double (*out)[ST], int t,
                                   bool two_minus) {
     const double sign = two_minus ? -1.0 : 1.0, diag = two_minus ? 1.0 : 0.0, two = 1.0 + 1.0;
+    double right[36];
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double v = sign * b[k * 6 + c][t];
+            if (k == c) v += diag * two;
+            right[k * 6 + c] = v;
+        }
+#pragma unroll
     for (int r = 0; r < 6; ++r) {
         double ar[6], res[6];
+#pragma unroll
         for (int k = 0; k < 6; ++k) ar[k] = a[r * 6 + k][t];
+#pragma unroll
         for (int c = 0; c < 6; ++c) {
             double s = 0.0;
-            for (int k = 0; k < 6; ++k) {
-                double right = sign * b[k * 6 + c][t];
-                if (k == c) right += diag * two;
-                s += ar[k] * right;
-            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) s += ar[k] * right[k * 6 + c];
             res[c] = 0.0 * out[r * 6 + c][t] + s;
         }
+#pragma unroll
         for (int c = 0; c < 6; ++c) out[r * 6 + c][t] = res[c];
     }
 }
@@ -75,7 +86,7 @@ __device__ inline void gemm_block(const double (*a)[ST], const double (*b)[ST], 
 // Midpoint, residual, mass block and residual ratio of one candidate velocity
 // (pre + gather + body_post + wnorm for one body without joints). Out of line: one
 // copy serves the iterate and every line-search lane; it touches no LDS.
-__device__ __noinline__ double evaluate_single(const double pose[8], const double v[6], double mass,
+__device__ __forceinline__ double evaluate_single(const double pose[8], const double v[6], double mass,
                                                const double tens[9], const double snap[6], const double ext[6],
                                                double half, double mid_out[8], double rhs_out[6],
                                                double block[6][6]) {
