@@ -1,14 +1,16 @@
-// One implicit Newton step of single-body islands without joints, one thread per
-// island: the control flow of island.hip (crates/newton/src/integrator/implicit/newton.rs:
-// 334-683, 209-327, 688-753) restated for a single 6×6 block, in the reference's own
-// summation order (the oracle's term order for gemm, reduce, matvec, seed).
+// One implicit Newton step of single-body islands without joints: the control flow of
+// island.hip (crates/newton/src/integrator/implicit/newton.rs:334-683, 209-327,
+// 688-753) restated for a single 6×6 block, in the reference's own summation order
+// (the oracle's term order for gemm, reduce, matvec, seed).
 //
-// Free bodies are most of the batched and served worlds (60 % of their bodies); a
-// workgroup per such island would leave 63 of 64 lanes idle. Here 64 islands share a
-// wave and each lane runs its island's whole step. The four 6×6 matrices of a lane
-// (A, X, T = scratch product, the candidate-0 mass block) live in LDS laid out
-// element-major ([36][64] per matrix), so the 64 lanes touch 64 consecutive doubles
-// at every access.
+// Free bodies are most of the batched and served worlds (60 % of their bodies). Eight
+// lanes serve one island: lane r (r < 6; lanes 6 and 7 mirror row 5 and store
+// nothing) holds row r of A, X, the product T of a Newton–Schulz iteration and the
+// candidate-0 mass block in registers and forms that row of every product; the other
+// rows come from the group by lane shuffles. The body's own arithmetic (midpoint,
+// residual, commit) runs in every lane on the same values, so the eight lanes take
+// the same decisions. A wave holds eight islands and takes as long as the slowest of
+// them: a body whose step subdivides holds up seven others, not sixty-three.
 //
 // Line-search lanes are evaluated in order with early exit: the first lane whose
 // residual drops below the current one is accepted, which is the lane the reference's
@@ -22,125 +24,167 @@ namespace exg {
 
 namespace {
 
-constexpr int ST = 64;           // threads (islands) per workgroup
+constexpr int SG = 8;            // lanes per island
+constexpr int SB = 256;          // threads per workgroup (32 islands)
 constexpr int MAX_IT = 32;
 constexpr int DEPTH = 8;
 constexpr int THRESH = 8;
 constexpr int PER_HALVING = 4;
 constexpr int NLANES = 8;
 
-struct SingleShared {
-    double A[36][ST];
-    double X[36][ST];
-    double T[36][ST];
-    double M0[36][ST];   // candidate-0 (iterate) world mass block
-};
-
 __device__ inline double mx2(double x, double y) { return x > y ? x : y; }
+// `v` as held by lane j of this lane's group of eight
+__device__ __forceinline__ double from(double v, int j) { return __shfl(v, j, SG); }
+// entry `row` of a six-vector, by selects on the values (passed by value, so the
+// compiler cannot turn the selects into a dynamically indexed load from memory)
+__device__ __forceinline__ double pick6(double v0, double v1, double v2, double v3, double v4, double v5, int row) {
+    double out = v0;
+    out = row == 1 ? v1 : out;
+    out = row == 2 ? v2 : out;
+    out = row == 3 ? v3 : out;
+    out = row == 4 ? v4 : out;
+    out = row == 5 ? v5 : out;
+    return out;
+}
 
-// Σ over one block of (diag·δ − b)², from 0, row-major (fn_reduce_sq_term)
-__device__ inline double reduce_block(const double (*b)[ST], int t, double diag) {
+// Σ over the six rows held by the group (row r in lane r) of (diag·δ − b)², from 0,
+// row-major (fn_reduce_sq_term): every lane gathers the 36 terms and folds them in
+// order, so every lane holds the same sum.
+__device__ __forceinline__ double reduce_rows(const double br[6], double diag) {
     double out = 0.0;
+#pragma unroll
     for (int r = 0; r < 6; ++r)
+#pragma unroll
         for (int c = 0; c < 6; ++c) {
-            double v = b[r * 6 + c][t];
-            double e = (r == c) ? diag - v : -v;
+            const double v = from(br[c], r);
+            const double e = (r == c) ? diag - v : -v;
             out += e * e;
         }
     return 0.0 + out;
 }
 
-// out = 0·out + a·(sign·b + diag·2I)  (fn_gemm_term with one neighbour block),
-// a row at a time. `out` may be `a` (not `b`): row r of the result needs only row r
-// of `a`, which is read before the row is stored.
-// The right operand (sign·b + diag·2I) is formed once in registers.
-__device__ inline void gemm_block(const double (*a)[ST], const double (*b)[ST], double (*out)[ST], int t,
-                                  bool two_minus) {
-    const double sign = two_minus ? -1.0 : 1.0, diag = two_minus ? 1.0 : 0.0, two = 1.0 + 1.0;
-    double right[36];
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double v = sign * b[k * 6 + c][t];
-            if (k == c) v += diag * two;
-            right[k * 6 + c] = v;
-        }
-#pragma unroll
-    for (int r = 0; r < 6; ++r) {
-        double ar[6], res[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) ar[k] = a[r * 6 + k][t];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < 6; ++k) s += ar[k] * right[k * 6 + c];
-            res[c] = 0.0 * out[r * 6 + c][t] + s;
-        }
-#pragma unroll
-        for (int c = 0; c < 6; ++c) out[r * 6 + c][t] = res[c];
-    }
-}
-
-// Midpoint, residual, mass block and residual ratio of one candidate velocity
-// (pre + gather + body_post + wnorm for one body without joints). Out of line: one
-// copy serves the iterate and every line-search lane; it touches no LDS.
-__device__ __forceinline__ double evaluate_single(const double pose[8], const double v[6], double mass,
-                                               const double tens[9], const double snap[6], const double ext[6],
-                                               double half, double mid_out[8], double rhs_out[6],
-                                               double block[6][6]) {
-    double e[8], sc[2];
-    twist_exp(v, half, e);
-    compose(pose, e, mid_out);
-    const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-    body_post(mid_out, v, mass, tens, snap, ext, half, floor2, block, rhs_out, sc);
-    double lin2 = rhs_out[0] * rhs_out[0] + rhs_out[1] * rhs_out[1] + rhs_out[2] * rhs_out[2];
-    double ang2 = rhs_out[3] * rhs_out[3] + rhs_out[4] * rhs_out[4] + rhs_out[5] * rhs_out[5];
-    double worst = 0.0;
-    worst = mx2(worst, lin2 / sc[0]);
-    worst = mx2(worst, ang2 / sc[1]);
-    return worst;
-}
-
 struct Single {
-    SingleShared& s;
     const IslandArgs& a;
-    int t, isl;
+    int row, isl;
     double pose[8], mom[6], mass, tens[9], ext[6];
     double snap[6], vmid[6], mid[8], rhs[6], best_v[6], dv[6];
+    double Ar[6], Xr[6], Tr[6], Mr[6];  // row `row` of A, X, T and the candidate-0 mass block
     double dt, warp, half, span;
     bool seeded;
     int st_spans = 0, st_newton = 0, st_ns = 0, st_reseed = 0, st_reject = 0, st_uncommit = 0;
 
-    // midpoint, residual and mass block of a candidate velocity (wnorm: newton.rs:139-151)
-    __device__ double evaluate(const double v[6], double mid_out[8], double rhs_out[6], double block[6][6]) {
-        return evaluate_single(pose, v, mass, tens, snap, ext, half, mid_out, rhs_out, block);
-    }
-    __device__ void store_m0(const double block[6][6]) {
-        for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) s.M0[r * 6 + c][t] = block[r][c];
-    }
-    __device__ void seed_transpose() {  // newton.rs:209-244
-        double row = 0.0, col = 0.0;
-        for (int e = 0; e < 36; ++e) {
-            row += fabs(s.A[e][t]);
-            col += fabs(s.A[e][t]);
+    // Midpoint, residual and residual ratio of a candidate velocity (pre + gather +
+    // body_post + wnorm for one body without joints; functions/src/lib.rs:170-247,
+    // newton.rs:139-151), and row `row` of its world mass block Ad*_M ∘ I ∘ Ad_{M⁻¹}.
+    // Each block entry is formed as body_post forms it; the momentum P = block·V
+    // gathers its rows from the group.
+    __device__ __forceinline__ double evaluate(const double v[6], double mid_out[8], double rhs_out[6], double brow[6]) {
+        double e[8];
+        twist_exp(v, half, e);
+        compose(pose, e, mid_out);
+        double inv[8], adi[6][6], cad[6][6];
+        motor_rev(mid_out, inv);
+        ad_matrix(inv, adi);
+        coad_matrix(mid_out, cad);
+        double cr[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) cr[k] = pick6(cad[0][k], cad[1][k], cad[2][k], cad[3][k], cad[4][k], cad[5][k], row);
+#pragma unroll
+        for (int col = 0; col < 6; ++col) {
+            double x[6];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) x[i] = adi[i][col] * mass;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                x[3 + r] = tens[3 * r] * adi[3][col] + tens[3 * r + 1] * adi[4][col] + tens[3 * r + 2] * adi[5][col];
+            double acc = cr[0] * x[0];
+#pragma unroll
+            for (int k = 1; k < 6; ++k) acc = acc + cr[k] * x[k];
+            brow[col] = acc;
         }
-        double inf = mx2(0.0, row), one = mx2(0.0, col);
-        double scale = 1.0 / (one * inf);
+        double pr = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) pr = pr + brow[c] * v[c];
+        double p[6], hh[6];
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            p[r] = from(pr, r);
+            hh[r] = ext[r] * half;
+            rhs_out[r] = -(p[r] - snap[r] - hh[r]);
+        }
+        const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
+        double sc[2];
+#pragma unroll
+        for (int o = 0; o < 2; ++o) {
+            const int b = 3 * o;
+            const double sp_ = p[b] * p[b] + p[b + 1] * p[b + 1] + p[b + 2] * p[b + 2];
+            const double sn = snap[b] * snap[b] + snap[b + 1] * snap[b + 1] + snap[b + 2] * snap[b + 2];
+            const double sh = hh[b] * hh[b] + hh[b + 1] * hh[b + 1] + hh[b + 2] * hh[b + 2];
+            sc[o] = fmax(fmax(fmax(sp_, sn), sh), floor2);
+        }
+        const double lin2 = rhs_out[0] * rhs_out[0] + rhs_out[1] * rhs_out[1] + rhs_out[2] * rhs_out[2];
+        const double ang2 = rhs_out[3] * rhs_out[3] + rhs_out[4] * rhs_out[4] + rhs_out[5] * rhs_out[5];
+        double worst = 0.0;
+        worst = mx2(worst, lin2 / sc[0]);
+        worst = mx2(worst, ang2 / sc[1]);
+        return worst;
+    }
+    __device__ __forceinline__ void seed_transpose() {  // newton.rs:209-244, one block: X = Aᵀ/(‖A‖₁‖A‖∞)
+        double row_s = 0.0, col_s = 0.0;
+#pragma unroll
         for (int r = 0; r < 6; ++r)
-            for (int c = 0; c < 6; ++c) s.X[r * 6 + c][t] = s.A[c * 6 + r][t] * scale;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const double v = fabs(from(Ar[c], r));
+                row_s += v;
+                col_s += v;
+            }
+        const double inf = mx2(0.0, row_s), one = mx2(0.0, col_s);
+        const double scale = 1.0 / (one * inf);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {  // X[row][c] = A[c][row]·scale
+            Xr[c] = pick6(from(Ar[0], c), from(Ar[1], c), from(Ar[2], c), from(Ar[3], c), from(Ar[4], c),
+                          from(Ar[5], c), row) * scale;
+        }
         seeded = true;
     }
-    __device__ double refine_inverse() {  // newton.rs:255-279
-        gemm_block(s.A, s.X, s.T, t, false);
-        double before = reduce_block(s.T, t, 1.0);
-        gemm_block(s.X, s.T, s.X, t, true);  // in place: X is the left operand
+    // T = 0·T + A·X, ‖I − T‖², X ← 0·X + X·(2I − T) (newton.rs:255-279), the products
+    // term for term as the oracle's gemm: out = 0·out + a·(sign·b + diag·2I).
+    __device__ __forceinline__ double refine_inverse() {
+        const double two = 1.0 + 1.0;
+        double tn[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                double right = 1.0 * from(Xr[c], k);
+                if (k == c) right += 0.0 * two;
+                s += Ar[k] * right;
+            }
+            tn[c] = 0.0 * Tr[c] + s;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Tr[c] = tn[c];
+        const double before = reduce_rows(Tr, 1.0);
+        double xn[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                double right = -1.0 * from(Tr[c], k);
+                if (k == c) right += 1.0 * two;
+                s += Xr[k] * right;
+            }
+            xn[c] = 0.0 * Xr[c] + s;
+        }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Xr[c] = xn[c];
         st_ns++;
         return before;
     }
-    __device__ void run_budget(int budget, int seed_budget) {  // newton.rs:301-327
+    __device__ __forceinline__ void run_budget(int budget, int seed_budget) {  // newton.rs:301-327
         bool has_prev = false, reseeded = false;
         double previous = 0.0;
         int remaining = budget;
@@ -159,7 +203,7 @@ struct Single {
             has_prev = true;
         }
     }
-    __device__ void newton_solve(double tol2, int ns_warm, int ns_seed, bool& converged, double& best,
+    __device__ __forceinline__ void newton_solve(double tol2, int ns_warm, int ns_seed, bool& converged, double& best,
                                  bool& assembled) {  // newton.rs:334-530
         const double huge = 1000000000.0;
         world_velocity(pose, mass, tens, mom, vmid);
@@ -176,9 +220,7 @@ struct Single {
                 has_carried = false;
                 rnorm = carried;
             } else {
-                double block[6][6];
-                rnorm = evaluate(vmid, mid, rhs, block);
-                store_m0(block);
+                rnorm = evaluate(vmid, mid, rhs, Mr);
             }
             if (rnorm < best) {
                 best = rnorm;
@@ -190,7 +232,7 @@ struct Single {
             }
             if (!(rnorm < huge)) break;
             // assemble: A = 0·A + 1·mass block (no joint terms)
-            for (int e = 0; e < 36; ++e) s.A[e][t] = 0.0 * s.A[e][t] + 1.0 * s.M0[e][t];
+            for (int c = 0; c < 6; ++c) Ar[c] = 0.0 * Ar[c] + 1.0 * Mr[c];
             assembled = true;
             int budget;
             if (seeded) budget = ns_warm;
@@ -199,11 +241,13 @@ struct Single {
                 budget = ns_seed;
             }
             run_budget(budget, ns_seed);
+            // dv = 0·dv + X·rhs: row `row` here, the others from the group
+            double sum = 0.0;
+            for (int c = 0; c < 6; ++c) sum += Xr[c] * rhs[c];
+            const double mine = 0.0 * pick6(dv[0], dv[1], dv[2], dv[3], dv[4], dv[5], row) + sum;
             bool nz = false;
-            for (int r = 0; r < 6; ++r) {  // dv = 0·dv + X·rhs
-                double sum = 0.0;
-                for (int c = 0; c < 6; ++c) sum += s.X[r * 6 + c][t] * rhs[c];
-                dv[r] = 0.0 * dv[r] + sum;
+            for (int r = 0; r < 6; ++r) {
+                dv[r] = from(mine, r);
                 nz |= dv[r] != 0.0;
             }
             if (!nz) break;
@@ -212,11 +256,12 @@ struct Single {
             int accepted = -1;
             double tnorm = 0.0;
             for (int l = 0; l < NLANES; ++l) {
-                double v[6], m_l[8], r_l[6], block[6][6];
+                double v[6], m_l[8], r_l[6], brow[6];
                 const double alpha = ldexp(1.0, -l);
                 for (int k = 0; k < 6; ++k) v[k] = base[k] + dv[k] * alpha;
-                double w = evaluate(v, m_l, r_l, block);
-                if (l == 0) store_m0(block);  // the iterate's storage holds lane 0 from here on
+                const double w = evaluate(v, m_l, r_l, brow);
+                if (l == 0)
+                    for (int c = 0; c < 6; ++c) Mr[c] = brow[c];  // the iterate's block is lane 0's from here on
                 if (w < rnorm) {
                     accepted = l;
                     tnorm = w;
@@ -225,7 +270,8 @@ struct Single {
                         rhs[k] = r_l[k];
                     }
                     for (int k = 0; k < 8; ++k) mid[k] = m_l[k];
-                    if (l > 0) store_m0(block);  // adopt_lane (cache.rs:1115-1136)
+                    if (l > 0)
+                        for (int c = 0; c < 6; ++c) Mr[c] = brow[c];  // adopt_lane (cache.rs:1115-1136)
                     break;
                 }
             }
@@ -255,15 +301,18 @@ struct Single {
         }
         return true;
     }
-    __device__ bool signal_log2(bool assembled, int& r) {  // newton.rs:688-702
+    __device__ __forceinline__ bool signal_log2(bool assembled, int& r) {  // newton.rs:688-702
         if (!assembled) return false;
-        double an = 0.0 + reduce_block(s.A, t, 0.0);
-        double mn = 0.0 + reduce_block(s.M0, t, 0.0);
-        double stiff = effective_zero(mn) ? 0.0 : an / mn;
+        const double an = 0.0 + reduce_rows(Ar, 0.0);
+        const double mn = 0.0 + reduce_rows(Mr, 0.0);
+        const double stiff = effective_zero(mn) ? 0.0 : an / mn;
         return log2_floor(stiff, r);
     }
     static __device__ int floor_div(int q, int d) { return (q >= 0) ? q / d : -((-q + d - 1) / d); }
-    __device__ void solve_step(double tol2, int ns_warm, int ns_seed) {  // newton.rs:543-683
+    __device__ __forceinline__ void load_x(long long xoff) {
+        for (int c = 0; c < 6; ++c) Xr[c] = a.xacc[xoff + row * 6 + c];
+    }
+    __device__ __forceinline__ void solve_step(double tol2, int ns_warm, int ns_seed, bool writer) {  // newton.rs:543-683
         const long long xoff = a.xacc_off[isl];
         bool accepted_seeded = seeded;
         uint64_t remaining = 1ull << DEPTH;
@@ -282,15 +331,15 @@ struct Single {
             bool converged, assembled;
             double best;
             newton_solve(tol2, ns_warm, ns_seed, converged, best, assembled);
-            bool finite = best < 1000000000.0;
-            bool last = depth == DEPTH;
+            const bool finite = best < 1000000000.0;
+            const bool last = depth == DEPTH;
             if (!converged && !last) {
-                for (int e = 0; e < 36; ++e) s.X[e][t] = a.xacc[xoff + e];  // restore X_acc
+                load_x(xoff);  // restore X_acc
                 seeded = accepted_seeded;
                 int r, down;
                 if (signal_log2(assembled, r)) down = floor_div(r - THRESH, PER_HALVING) + 1;
                 else down = 1;
-                int hi = DEPTH - depth;
+                const int hi = DEPTH - depth;
                 down = down < 1 ? 1 : (down > hi ? hi : down);
                 depth += down;
                 continue;
@@ -298,8 +347,9 @@ struct Single {
             if (converged || finite) {
                 if (!converged) st_uncommit++;
                 commit_body(pose, mom, mass, tens, vmid, snap, span, half, converged);
-                for (int e = 0; e < 36; ++e) a.xacc[xoff + e] = s.X[e][t];  // publish X_acc
-                double nx = 0.0 + reduce_block(s.X, t, 0.0);
+                if (writer)
+                    for (int c = 0; c < 6; ++c) a.xacc[xoff + row * 6 + c] = Xr[c];  // publish X_acc
+                const double nx = 0.0 + reduce_rows(Xr, 0.0);
                 const double huge = 1000000000.0;
                 if (nx < huge * huge) accepted_seeded = seeded;
                 else {
@@ -307,7 +357,7 @@ struct Single {
                     accepted_seeded = false;
                 }
             } else {
-                for (int e = 0; e < 36; ++e) s.X[e][t] = a.xacc[xoff + e];
+                load_x(xoff);
                 seeded = accepted_seeded;
             }
             remaining -= units;
@@ -328,40 +378,39 @@ struct Single {
     }
 };
 
-__global__ __launch_bounds__(ST) void singles_step_kernel(IslandArgs a) {
-    __shared__ SingleShared s;
-    const int t = threadIdx.x;
-    const int k = blockIdx.x * ST + t;
-    if (k >= a.n_islands) return;  // no barriers below: each lane is independent
+__global__ __launch_bounds__(SB) void singles_step_kernel(IslandArgs a) {
+    const int j = threadIdx.x & (SG - 1);
+    const int k = (int)((blockIdx.x * SB + threadIdx.x) / SG);
+    // whole groups leave together: the eight lanes of a group share k
+    if (k >= a.n_islands) return;
     const int isl = a.island_ids ? a.island_ids[k] : k;
     const int b = a.island_start[isl];
     if (a.island_end[isl] - b != 1 || a.island_jend[isl] != a.island_jstart[isl]) return;  // routed by class; never taken
-    Single sg{s, a, t, isl};
+    const bool writer = j < 6;  // lanes 6 and 7 mirror row 5
+    Single sg{a, j < 6 ? j : 5, isl};
     for (int q = 0; q < 8; ++q) sg.pose[q] = a.pose[(size_t)b * 8 + q];
     for (int q = 0; q < 6; ++q) {
         sg.mom[q] = a.mom[(size_t)b * 6 + q];
         sg.ext[q] = a.ext[(size_t)b * 6 + q];
         sg.dv[q] = 0.0;
+        sg.Ar[q] = 0.0;
+        sg.Tr[q] = 0.0;
+        sg.Mr[q] = 0.0;
     }
     for (int q = 0; q < 9; ++q) sg.tens[q] = a.tens[(size_t)b * 9 + q];
     sg.mass = a.mass[b];
-    const long long xoff = a.xacc_off[isl];
-    for (int e = 0; e < 36; ++e) {
-        s.A[e][t] = 0.0;
-        s.T[e][t] = 0.0;
-        s.M0[e][t] = 0.0;
-        s.X[e][t] = a.xacc[xoff + e];
-    }
+    sg.load_x(a.xacc_off[isl]);
     sg.dt = a.dt;
     sg.warp = a.warp;
     sg.seeded = a.seeded[isl] != 0;
-    sg.solve_step(a.tol2, a.ns_warm, a.ns_seed);
+    sg.solve_step(a.tol2, a.ns_warm, a.ns_seed, writer);
 
     // recompute_centroid (island.rs:206-244) for one body, then the charge
     double p[3];
     body_position(sg.pose, p);
     const double total_m = sg.mass;
     double wx[3] = {0.0 + p[0] * sg.mass, 0.0 + p[1] * sg.mass, 0.0 + p[2] * sg.mass};
+    double org[3] = {a.origin[isl * 3 + 0], a.origin[isl * 3 + 1], a.origin[isl * 3 + 2]};
     if (!effective_zero(total_m)) {
         double inv = 1.0 / total_m;
         double sh[6] = {-(wx[0] * inv), -(wx[1] * inv), -(wx[2] * inv), 0.0, 0.0, 0.0};
@@ -369,12 +418,14 @@ __global__ __launch_bounds__(ST) void singles_step_kernel(IslandArgs a) {
         twist_exp(sh, 1.0, shift);
         compose(sg.pose, shift, np);
         for (int q = 0; q < 8; ++q) sg.pose[q] = np[q];
-        for (int q = 0; q < 3; ++q) a.origin[isl * 3 + q] += wx[q] * inv;
+        for (int q = 0; q < 3; ++q) org[q] += wx[q] * inv;
     }
+    if (j != 0) return;
+    for (int q = 0; q < 3; ++q) a.origin[isl * 3 + q] = org[q];
     body_position(sg.pose, p);
     double* c = a.charges_out + (size_t)b * 8;
     for (int q = 0; q < 3; ++q) {
-        c[q] = p[q] + a.origin[isl * 3 + q];
+        c[q] = p[q] + org[q];
         c[3 + q] = p[q];
     }
     c[6] = (a.G_of_island ? a.G_of_island[isl] : a.G) * sg.mass;
@@ -395,7 +446,8 @@ __global__ __launch_bounds__(ST) void singles_step_kernel(IslandArgs a) {
 
 void launch_singles(const IslandArgs& a, hipStream_t stream) {
     if (a.n_islands <= 0) return;
-    hipLaunchKernelGGL(singles_step_kernel, dim3((a.n_islands + ST - 1) / ST), dim3(ST), 0, stream, a);
+    const long long lanes = (long long)a.n_islands * SG;
+    hipLaunchKernelGGL(singles_step_kernel, dim3((unsigned)((lanes + SB - 1) / SB)), dim3(SB), 0, stream, a);
 }
 
 }  // namespace exg
