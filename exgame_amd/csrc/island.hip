@@ -22,6 +22,7 @@
 #include <type_traits>
 #include "island.h"
 #include "physics_dev.h"
+#include "body_rows.h"
 
 namespace exg {
 
@@ -183,57 +184,63 @@ struct IslandSolver {
                          VMID(lane, s.jb[j]), dt, warp, CONN(lane, j, 0), CONN(lane, j, 1));
         }
     }
-    // ───── gather (external + incident wrenches in joint order) + body post ─────
-    __device__ void gather_post_one(int lane, int i, bool write_mass) {
-        double tot[6];
-        for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
-        for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
-            int code = s.inc[q];
-            const double* w = CONN(lane, code >> 1, code & 1);
-            for (int k = 0; k < 6; ++k) tot[k] = tot[k] + w[k];
-        }
-        double* tl = TOTAL(lane, i);
-        for (int k = 0; k < 6; ++k) tl[k] = tot[k];
-        double block[6][6];
-        const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-        body_post(MID(lane, i), VMID(lane, i), s.massv[i], s.tens[i], s.snap[i], tot, half, floor2, block,
-                  RHS(lane, i), SCALE(lane, i));
-        if (write_mass)
-            for (int r = 0; r < 6; ++r)
-                for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = block[r][c];
-    }
     static __device__ double max2(double x, double y) { return x > y ? x : y; }
-    // Gather + body post of lanes [l0, l1), one thread per (lane, body), and the worst²
-    // residual ratio of each lane (newton.rs:139-151) into s.wn (visible after the
-    // caller's barrier). Every (lane, body) thread sits in wave 0: each forms its two
-    // ratios, and the lane's thread folds its bodies' ratios in body order, reading
-    // them across the wave with the same max2 (NaN handling as the sequential form).
+    // ───── gather (external + incident wrenches in joint order) + body post + wnorm ─────
+    // Lanes [l0, l1), eight threads per (lane, body): lane l's bodies take threads
+    // [l·LSTR, l·LSTR + 8m) with LSTR = 8·MAXM, so a wave holds whole lanes. The eight
+    // threads of a body form one row each of its world mass block (body_rows.h) and
+    // end with the same residual; the worst² residual ratio of each lane
+    // (newton.rs:139-151) is folded across the wave in body order with max2 (NaN
+    // handling as the sequential form) into s.wn (visible after the caller's barrier).
     // Candidate 0's world mass block goes to mass0 only when mass0 may take it: in
     // the second line-search round candidate 0 is lane 4, and mass0 must keep lane
     // 0's block unless a later lane is accepted (the oracle's lanes[0].mass).
     __device__ void gather_post(int l0, int l1, bool mass_of_candidate0 = true) {
-        static_assert(LS * MAXM <= 64, "a line-search round's (lane, body) pairs fit one wave");
+        constexpr int LSTR = 8 * MAXM;
+        static_assert(64 % LSTR == 0 && LS * LSTR <= NTH, "a wave holds whole line-search lanes");
         const int nl = l1 - l0, t = threadIdx.x;
+        const int ll = t / LSTR, i = (t % LSTR) >> 3, j = t & 7;
         double r0 = 0.0, r1 = 0.0;
-        if (t < nl * m) {
-            const int lane = l0 + t / m, i = t % m;
-            gather_post_one(lane, i, lane == 0 && mass_of_candidate0);
-            const double* r = RHS(lane, i);
-            const double* sc = SCALE(lane, i);
-            const double lin2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            const double ang2 = r[3] * r[3] + r[4] * r[4] + r[5] * r[5];
+        if (ll < nl && i < m) {
+            const int lane = l0 + ll;
+            double tot[6];
+            for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
+            for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
+                const int code = s.inc[q];
+                const double* w = CONN(lane, code >> 1, code & 1);
+                for (int k = 0; k < 6; ++k) tot[k] = tot[k] + w[k];
+            }
+            const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
+            double brow[6], rhs[6], sc[2];
+            body_post_row(MID(lane, i), VMID(lane, i), s.massv[i], s.tens[i], s.snap[i], tot, half, floor2,
+                          j < 6 ? j : 5, brow, rhs, sc);
+            if (j == 0) {
+                double* tl = TOTAL(lane, i);
+                double* rh = RHS(lane, i);
+                double* scl = SCALE(lane, i);
+                for (int k = 0; k < 6; ++k) {
+                    tl[k] = tot[k];
+                    rh[k] = rhs[k];
+                }
+                scl[0] = sc[0];
+                scl[1] = sc[1];
+            }
+            if (lane == 0 && mass_of_candidate0 && j < 6)
+                for (int c = 0; c < 6; ++c) s.mass0[i][j * 6 + c] = brow[c];
+            const double lin2 = rhs[0] * rhs[0] + rhs[1] * rhs[1] + rhs[2] * rhs[2];
+            const double ang2 = rhs[3] * rhs[3] + rhs[4] * rhs[4] + rhs[5] * rhs[5];
             r0 = lin2 / sc[0];
             r1 = ang2 / sc[1];
         }
-        if (t < 64) {
-            const int me = t < nl ? t : 0;
+        {   // every lane of every wave runs the shuffles
+            const int base = (t & 63) & ~(LSTR - 1);
             double worst = 0.0;
-            for (int i = 0; i < m; ++i) {
-                const double a0 = __shfl(r0, me * m + i, 64), a1 = __shfl(r1, me * m + i, 64);
+            for (int q = 0; q < m; ++q) {
+                const double a0 = __shfl(r0, base + q * 8, 64), a1 = __shfl(r1, base + q * 8, 64);
                 worst = max2(worst, a0);
                 worst = max2(worst, a1);
             }
-            if (t < nl) s.wn[l0 + t] = worst;
+            if ((t % LSTR) == 0 && ll < nl) s.wn[l0 + ll] = worst;
         }
     }
     // ───── system matrix: A = 0·A + [i==j]·𝕀 + Σ terms (pf·∂W/∂pose + vf·∂W/∂vel) ─────

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 #include "island.h"
 #include "physics_dev.h"
+#include "body_rows.h"
 
 namespace exg {
 
@@ -33,19 +34,7 @@ constexpr int PER_HALVING = 4;
 constexpr int NLANES = 8;
 
 __device__ inline double mx2(double x, double y) { return x > y ? x : y; }
-// `v` as held by lane j of this lane's group of eight
-__device__ __forceinline__ double from(double v, int j) { return __shfl(v, j, SG); }
-// entry `row` of a six-vector, by selects on the values (passed by value, so the
-// compiler cannot turn the selects into a dynamically indexed load from memory)
-__device__ __forceinline__ double pick6(double v0, double v1, double v2, double v3, double v4, double v5, int row) {
-    double out = v0;
-    out = row == 1 ? v1 : out;
-    out = row == 2 ? v2 : out;
-    out = row == 3 ? v3 : out;
-    out = row == 4 ? v4 : out;
-    out = row == 5 ? v5 : out;
-    return out;
-}
+__device__ __forceinline__ double from(double v, int j) { return group8(v, j); }
 
 // Σ over the six rows held by the group (row r in lane r) of (diag·δ − b)², from 0,
 // row-major (fn_reduce_sq_term): every lane gathers the 36 terms and folds them in
@@ -79,49 +68,11 @@ struct Single {
     // Each block entry is formed as body_post forms it; the momentum P = block·V
     // gathers its rows from the group.
     __device__ __forceinline__ double evaluate(const double v[6], double mid_out[8], double rhs_out[6], double brow[6]) {
-        double e[8];
+        double e[8], sc[2];
         twist_exp(v, half, e);
         compose(pose, e, mid_out);
-        double inv[8], adi[6][6], cad[6][6];
-        motor_rev(mid_out, inv);
-        ad_matrix(inv, adi);
-        coad_matrix(mid_out, cad);
-        double cr[6];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) cr[k] = pick6(cad[0][k], cad[1][k], cad[2][k], cad[3][k], cad[4][k], cad[5][k], row);
-#pragma unroll
-        for (int col = 0; col < 6; ++col) {
-            double x[6];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) x[i] = adi[i][col] * mass;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-                x[3 + r] = tens[3 * r] * adi[3][col] + tens[3 * r + 1] * adi[4][col] + tens[3 * r + 2] * adi[5][col];
-            double acc = cr[0] * x[0];
-#pragma unroll
-            for (int k = 1; k < 6; ++k) acc = acc + cr[k] * x[k];
-            brow[col] = acc;
-        }
-        double pr = 0.0;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) pr = pr + brow[c] * v[c];
-        double p[6], hh[6];
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
-            p[r] = from(pr, r);
-            hh[r] = ext[r] * half;
-            rhs_out[r] = -(p[r] - snap[r] - hh[r]);
-        }
         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-        double sc[2];
-#pragma unroll
-        for (int o = 0; o < 2; ++o) {
-            const int b = 3 * o;
-            const double sp_ = p[b] * p[b] + p[b + 1] * p[b + 1] + p[b + 2] * p[b + 2];
-            const double sn = snap[b] * snap[b] + snap[b + 1] * snap[b + 1] + snap[b + 2] * snap[b + 2];
-            const double sh = hh[b] * hh[b] + hh[b + 1] * hh[b + 1] + hh[b + 2] * hh[b + 2];
-            sc[o] = fmax(fmax(fmax(sp_, sn), sh), floor2);
-        }
+        body_post_row(mid_out, v, mass, tens, snap, ext, half, floor2, row, brow, rhs_out, sc);
         const double lin2 = rhs_out[0] * rhs_out[0] + rhs_out[1] * rhs_out[1] + rhs_out[2] * rhs_out[2];
         const double ang2 = rhs_out[3] * rhs_out[3] + rhs_out[4] * rhs_out[4] + rhs_out[5] * rhs_out[5];
         double worst = 0.0;
