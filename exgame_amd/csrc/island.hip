@@ -158,20 +158,53 @@ struct IslandSolver {
         }
     }
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
+    // item q < 12J: pose column q % 12 of joint q / 12 (in kind order); item 12J + q:
+    // velocity column 12 + q % 12 of the same joint
+    __device__ void jacobian_item(int item) {
+        const int pose_item = item < J * 12;
+        const int q = pose_item ? item : item - J * 12;
+        const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
+        // the f64 lanes also deliver the wrench values; with f32 lanes the residual's
+        // wrenches come from an f64 value-only evaluation instead
+        const bool values = !JF && ax == 0;
+        joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp,
+                     s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
+                     values ? s.conn0[j][1] : nullptr);
+    }
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
-        // pose columns (tangents through the geometry) first, then the cheaper velocity
-        // columns: a thread that takes a second column takes a cheap one
-        for (int t = threadIdx.x; t < J * 24; t += NTH) {
-            const int pose_item = t < J * 12;
-            const int q = pose_item ? t : t - J * 12;
-            const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
-            // the f64 lanes also deliver the wrench values; with f32 lanes the residual's
-            // wrenches come from an f64 value-only evaluation instead
-            const bool values = !JF && ax == 0;
-            joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp,
-                         s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
-                         values ? s.conn0[j][1] : nullptr);
+        const int P = J * 12, t = threadIdx.x;
+        if constexpr (!JF) {
+            // pose columns first, then velocity columns: a thread that takes a second
+            // column takes a cheap one (the mapping below measured slower for f64 lanes)
+            for (int t2 = t; t2 < J * 24; t2 += NTH) {
+                const int pose_item = t2 < J * 12;
+                const int q = pose_item ? t2 : t2 - J * 12;
+                const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
+                // the f64 lanes also deliver the wrench values
+                const bool values = ax == 0;
+                joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt,
+                             warp, s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
+                             values ? s.conn0[j][1] : nullptr);
+            }
+            return;
+        }
+        int first, second;
+        if (3 * P <= 2 * NTH) {
+            // one pose column (tangents through the geometry) per thread; the cheaper
+            // velocity columns, at most two per thread, on the remaining threads
+            first = t < P ? t : (t - P < P ? t : -1);
+            second = (t >= P && t + NTH - P < 2 * P) ? t + NTH - P : -1;
+        } else {
+            // pose columns first, then velocity columns: a thread that takes a second
+            // column takes a cheap one (2P ≤ 2·NTH: at most two per thread)
+            first = t < 2 * P ? t : -1;
+            second = t + NTH < 2 * P ? t + NTH : -1;
+        }
+#pragma clang loop unroll(disable)
+        for (int pass = 0; pass < 2; ++pass) {  // one call site: the column code is inlined once
+            const int item = pass ? second : first;
+            if (item >= 0) jacobian_item(item);
         }
     }
     // ───── value-only joints for lanes [l0, l1) ─────
