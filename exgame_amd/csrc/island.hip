@@ -157,6 +157,21 @@ struct IslandSolver {
             compose(s.pose[i], e, MID(lane, i));
         }
     }
+    // line-search round `round`: candidate `slot` takes vmid = base + 2^-(round·LS + slot)·dv
+    // (newton.rs:469-522) and its midpoint in the same pass
+    __device__ void pre_candidates(int round) {
+        for (int t = threadIdx.x; t < LS * m; t += NTH) {
+            const int slot = t / m, i = t % m;
+            const double alpha = ldexp(1.0, -(round * LS + slot));
+            double v[6], e[8];
+            for (int k = 0; k < 6; ++k) {
+                v[k] = s.base[i][k] + s.dv[i][k] * alpha;
+                VMID(slot, i)[k] = v[k];
+            }
+            twist_exp(v, half, e);
+            compose(s.pose[i], e, MID(slot, i));
+        }
+    }
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
     // item q < 12J: pose column q % 12 of joint q / 12 (in kind order); item 12J + q:
     // velocity column 12 + q % 12 of the same joint
@@ -635,12 +650,7 @@ struct IslandSolver {
             int accepted = -1, accepted_slot = -1;
             double tnorm = 0.0;
             for (int round = 0; round < LANES / LS && accepted < 0; ++round) {
-                for (int t = threadIdx.x; t < LS * m * 6; t += NTH) {
-                    int slot = t / (m * 6), r = t % (m * 6), i = r / 6, k = r % 6;
-                    VMID(slot, i)[k] = s.base[i][k] + s.dv[i][k] * ldexp(1.0, -(round * LS + slot));
-                }
-                __syncthreads();
-                pre(0, LS);
+                pre_candidates(round);
                 __syncthreads();
                 joints_plain(0, LS);
                 __syncthreads();
