@@ -136,6 +136,13 @@ struct IslandSolver {
     int st_spans, st_newton, st_ns, st_reseed, st_reject, st_uncommit;
     bool seeded;
 
+    // the thread index as an opaque value at each use, so that index arithmetic is
+    // redone where it is needed instead of being kept live across the whole solve
+    static __device__ __forceinline__ int tid() {
+        int t = (int)threadIdx.x;
+        asm volatile("" : "+v"(t));
+        return t;
+    }
     __device__ double* A() { return s.A; }
     __device__ double* X() { return s.X; }
     __device__ double* R() { return s.u.T; }
@@ -150,7 +157,7 @@ struct IslandSolver {
 
     // ───── PRE: midpoint = pose ∘ exp(half·vmid) for lanes [l0, l1) ─────
     __device__ void pre(int l0, int l1) {
-        for (int t = threadIdx.x; t < (l1 - l0) * m; t += NTH) {
+        for (int t = tid(); t < (l1 - l0) * m; t += NTH) {
             int lane = l0 + t / m, i = t % m;
             double e[8];
             twist_exp(VMID(lane, i), half, e);
@@ -160,7 +167,7 @@ struct IslandSolver {
     // line-search round `round`: candidate `slot` takes vmid = base + 2^-(round·LS + slot)·dv
     // (newton.rs:469-522) and its midpoint in the same pass
     __device__ void pre_candidates(int round) {
-        for (int t = threadIdx.x; t < LS * m; t += NTH) {
+        for (int t = tid(); t < LS * m; t += NTH) {
             const int slot = t / m, i = t % m;
             const double alpha = ldexp(1.0, -(round * LS + slot));
             double v[6], e[8];
@@ -188,7 +195,7 @@ struct IslandSolver {
     }
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
-        const int P = J * 12, t = threadIdx.x;
+        const int P = J * 12, t = tid();
         if constexpr (!JF) {
             // pose columns first, then velocity columns: a thread that takes a second
             // column takes a cheap one (the mapping below measured slower for f64 lanes)
@@ -226,7 +233,7 @@ struct IslandSolver {
     __device__ void joints_plain(int l0, int l1) {
         if constexpr (MAXJ == 0) return;
         const int nl = l1 - l0;
-        for (int t = threadIdx.x; t < nl * J; t += NTH) {
+        for (int t = tid(); t < nl * J; t += NTH) {
             int lane = l0 + t % nl, j = s.jorder[t / nl];
             joint_wrench(s.jkind[j], s.jparams[j], MID(lane, s.ja[j]), MID(lane, s.jb[j]), VMID(lane, s.ja[j]),
                          VMID(lane, s.jb[j]), dt, warp, CONN(lane, j, 0), CONN(lane, j, 1));
@@ -246,7 +253,7 @@ struct IslandSolver {
     __device__ void gather_post(int l0, int l1, bool mass_of_candidate0 = true) {
         constexpr int LSTR = 8 * MAXM;
         static_assert(64 % LSTR == 0 && LS * LSTR <= NTH, "a wave holds whole line-search lanes");
-        const int nl = l1 - l0, t = threadIdx.x;
+        const int nl = l1 - l0, t = tid();
         const int ll = t / LSTR, i = (t % LSTR) >> 3, j = t & 7;
         double r0 = 0.0, r1 = 0.0;
         if (ll < nl && i < m) {
@@ -297,7 +304,7 @@ struct IslandSolver {
         double* Am = A();
         // the non-zero blocks only (neighbour lists); the others stay as the launch
         // zeroed them, which is what 0·A + 0 would write
-        for (int e = threadIdx.x; e < s.nbr_start[m] * 6; e += NTH) {
+        for (int e = tid(); e < s.nbr_start[m] * 6; e += NTH) {
             const int q = e / 6, r = e - 6 * q;
             int i = 0;
             while (s.nbr_start[i + 1] <= q) ++i;
@@ -348,8 +355,12 @@ struct IslandSolver {
         constexpr int MAXCELLS = (IslandShared<MAXM, MAXJ, NTH, JF>::MAXN * IslandShared<MAXM, MAXJ, NTH, JF>::MAXN + NTH - 1) / NTH;
         double cell[MAXCELLS];
         double sign = two_minus ? -1.0 : 1.0;
-        int u = 0;
-        for (int e = threadIdx.x; e < n * n; e += NTH, ++u) {
+        // constant cell indices (every loop unrolled), so that the cells are values of
+        // this call alone rather than an array whose contents outlive it
+#pragma unroll
+        for (int u = 0; u < MAXCELLS; ++u) {
+            const int e = tid() + u * NTH;
+            if (e >= n * n) break;
             int row = e / n, col = e % n, i = row / 6, j = col / 6, c = col % 6;
             double acc = 0.0 * out[row * LD + col];
             int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : m;
@@ -369,8 +380,12 @@ struct IslandSolver {
             cell[u] = acc;
         }
         if (in_place) __syncthreads();
-        u = 0;
-        for (int e = threadIdx.x; e < n * n; e += NTH, ++u) out[(e / n) * LD + e % n] = cell[u];
+#pragma unroll
+        for (int u = 0; u < MAXCELLS; ++u) {
+            const int e = tid() + u * NTH;
+            if (e >= n * n) break;
+            out[(e / n) * LD + e % n] = cell[u];
+        }
     }
     // ───── MFMA form: out = a·b over 16×16 tiles, k in 4-column slabs ─────
     // Wave w computes tiles w, w+4, … of the n16×n16 product; a slab whose bit is
@@ -384,7 +399,7 @@ struct IslandSolver {
     // their count of non-zero slabs), 1 for the dense one.
     __device__ double mfma_product(const double* am, const double* bm, double* out, const unsigned* kmask,
                                    bool make_t, bool in_place, int which) {
-        const int wave = uni((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+        const int wave = uni((int)(tid() >> 6)), lane = tid() & 63;
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
         const unsigned dense = (1u << KS) - 1u;
@@ -471,7 +486,7 @@ struct IslandSolver {
     // ───── Σ ‖diag·I − b‖² over the n×n entries of `mat` ─────
     __device__ double reduce_sq(const double* mat, bool identity_diag) {
         double part = 0.0;
-        for (int e = threadIdx.x; e < n * n; e += NTH) {
+        for (int e = tid(); e < n * n; e += NTH) {
             int row = e / n, col = e % n;
             double diag = (identity_diag && row == col) ? 1.0 : 0.0;
             double v = mat[row * LD + col];
@@ -499,19 +514,19 @@ struct IslandSolver {
     }
     __device__ void seed_transpose() {  // newton.rs:209-244
         // per block row i: Σ|A[i, ·]| and per block column i: Σ|A[·, i]|
-        if (threadIdx.x < 2 * m) {
-            int i = threadIdx.x % m;
-            bool col = threadIdx.x >= m;
+        if (tid() < 2 * m) {
+            int i = tid() % m;
+            bool col = tid() >= m;
             double sum = 0.0;
             const double* Am = A();
             for (int j = 0; j < m; ++j)
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c)
                         sum += fabs(col ? Am[(6 * j + r) * LD + 6 * i + c] : Am[(6 * i + r) * LD + 6 * j + c]);
-            s.seedsum[threadIdx.x] = sum;
+            s.seedsum[tid()] = sum;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tid() == 0) {
             double one = 0.0, inf = 0.0;
             for (int i = 0; i < m; ++i) {
                 inf = max2(inf, s.seedsum[i]);
@@ -522,7 +537,7 @@ struct IslandSolver {
         __syncthreads();
         double sc = uni(s.bcast[0]);
         double* Xm = X();
-        for (int e = threadIdx.x; e < n * n; e += NTH) {
+        for (int e = tid(); e < n * n; e += NTH) {
             int row = e / n, col = e % n;
             Xm[row * LD + col] = A()[col * LD + row] * sc;
         }
@@ -556,7 +571,7 @@ struct IslandSolver {
         const double* Xm = X();
         static_assert(4 * 6 * MAXM <= NTH, "four lanes per row of the largest island");
         if (mfma) {
-            const int t = threadIdx.x, row = t >> 2, part = t & 3;
+            const int t = tid(), row = t >> 2, part = t & 3;
             const bool live = t < 4 * n;
             double sum = 0.0;
             if (live)
@@ -571,7 +586,7 @@ struct IslandSolver {
             if (live && part == 0) s.dv[row / 6][row % 6] = 0.0 * s.dv[row / 6][row % 6] + sum;
             return;
         }
-        for (int t = threadIdx.x; t < n; t += NTH) {
+        for (int t = tid(); t < n; t += NTH) {
             int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
             for (int j = 0; j < m; ++j) {
@@ -584,7 +599,7 @@ struct IslandSolver {
         }
     }
     __device__ void copy_vmid(int dst, const double (*src)[6]) {
-        for (int t = threadIdx.x; t < m * 6; t += NTH) VMID(dst, t / 6)[t % 6] = src[t / 6][t % 6];
+        for (int t = tid(); t < m * 6; t += NTH) VMID(dst, t / 6)[t % 6] = src[t / 6][t % 6];
     }
 
     // ───── one Newton solve at the current `half` (newton.rs:334-530) ─────
@@ -593,10 +608,10 @@ struct IslandSolver {
         const double huge = 1000000000.0;
         // the iterate starts at each body's world velocity (the line-search candidates
         // are written from it before they are read)
-        for (int t = threadIdx.x; t < m; t += NTH)
+        for (int t = tid(); t < m; t += NTH)
             world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.vmid0[t]);
         __syncthreads();
-        for (int t = threadIdx.x; t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+        for (int t = tid(); t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
         __syncthreads();
         best = huge;
         converged = false;
@@ -623,7 +638,7 @@ struct IslandSolver {
             }
             if (rnorm < best) {
                 best = rnorm;
-                for (int t = threadIdx.x; t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+                for (int t = tid(); t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
             }
             if (rnorm <= tol2) { converged = true; break; }
             if (!(rnorm < huge)) break;
@@ -639,13 +654,13 @@ struct IslandSolver {
             // all-zero correction → hand over to subdivision
             {
                 int nz = 0;
-                for (int t = threadIdx.x; t < m * 6; t += NTH) nz |= (s.dv[t / 6][t % 6] != 0.0);
+                for (int t = tid(); t < m * 6; t += NTH) nz |= (s.dv[t / 6][t % 6] != 0.0);
                 if (!__syncthreads_or(nz)) break;
             }
             // backtracking line search from the same base, α = 2^-lane for lanes 0..7,
             // evaluated LS at a time; the first lane (in order) that lowers the
             // residual is taken, as when all eight are evaluated together
-            for (int t = threadIdx.x; t < m * 6; t += NTH) s.base[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+            for (int t = tid(); t < m * 6; t += NTH) s.base[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
             __syncthreads();
             int accepted = -1, accepted_slot = -1;
             double tnorm = 0.0;
@@ -668,14 +683,14 @@ struct IslandSolver {
                 if (accepted > 0) {  // adopt_lane (cache.rs:1115-1136)
                     const int as = accepted_slot;
                     if (as > 0) {
-                        for (int t = threadIdx.x; t < m * 6; t += NTH) {
+                        for (int t = tid(); t < m * 6; t += NTH) {
                             s.vmid0[t / 6][t % 6] = VMID(as, t / 6)[t % 6];
                             s.rhs0[t / 6][t % 6] = RHS(as, t / 6)[t % 6];
                         }
-                        for (int t = threadIdx.x; t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
+                        for (int t = tid(); t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
                     }
                     // the accepted lane's mass block, recomputed bit-identically from its inputs
-                    for (int t = threadIdx.x; t < m; t += NTH) {
+                    for (int t = tid(); t < m; t += NTH) {
                         double block[6][6], rr[6], sc[2];
                         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
                         body_post(MID(as, t), VMID(as, t), s.massv[t], s.tens[t], s.snap[t], TOTAL(as, t), half, floor2,
@@ -688,7 +703,7 @@ struct IslandSolver {
                 has_carried = true;
                 carried = tnorm;
             } else {
-                for (int t = threadIdx.x; t < m * 6; t += NTH) s.vmid0[t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
+                for (int t = tid(); t < m * 6; t += NTH) s.vmid0[t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
                 __syncthreads();
                 st_reject++;
                 break;
@@ -720,7 +735,7 @@ struct IslandSolver {
         double mn = 0.0;
         {
             double part = 0.0;
-            for (int e = threadIdx.x; e < m * 36; e += NTH) {
+            for (int e = tid(); e < m * 36; e += NTH) {
                 double v = s.mass0[e / 36][e % 36];
                 part = part + v * v;
             }
@@ -732,13 +747,13 @@ struct IslandSolver {
     __device__ void restore_x() {
         double* Xm = X();
         const double* src = a.xacc + xoff;
-        for (int e = threadIdx.x; e < n * n; e += NTH) Xm[(e / n) * LD + e % n] = src[e];
+        for (int e = tid(); e < n * n; e += NTH) Xm[(e / n) * LD + e % n] = src[e];
         __syncthreads();
     }
     __device__ void publish_x(bool& accepted_seeded) {
         const double* Xm = X();
         double* dst = a.xacc + xoff;
-        for (int e = threadIdx.x; e < n * n; e += NTH) dst[e] = Xm[(e / n) * LD + e % n];
+        for (int e = tid(); e < n * n; e += NTH) dst[e] = Xm[(e / n) * LD + e % n];
         double nx = reduce_sq(Xm, false);
         const double huge = 1000000000.0;
         if (nx < huge * huge) accepted_seeded = seeded;
@@ -758,7 +773,7 @@ struct IslandSolver {
             for (int k = 0; k < depth; ++k) span = span / 2.0;
             half = span / 2.0;
             st_spans++;
-            for (int t = threadIdx.x; t < m; t += NTH) world_momentum(s.pose[t], s.mom[t], s.snap[t]);
+            for (int t = tid(); t < m; t += NTH) world_momentum(s.pose[t], s.mom[t], s.snap[t]);
             __syncthreads();
             bool converged, assembled;
             double best;
@@ -778,7 +793,7 @@ struct IslandSolver {
             }
             if (converged || finite) {
                 if (!converged) st_uncommit++;
-                for (int t = threadIdx.x; t < m; t += NTH)
+                for (int t = tid(); t < m; t += NTH)
                     commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.vmid0[t], s.snap[t], span, half,
                                 converged);
                 __syncthreads();

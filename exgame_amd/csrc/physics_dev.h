@@ -303,8 +303,8 @@ __host__ __device__ inline void joint_wrench(int kind, const double* params, con
 // derivative gp(B_k, base), B_k the screw basis of the pose axis (ad.rs:119-150).
 template <class T>
 __host__ __device__ inline void seed_pose(const T base[8], int dof, Dual<T> out[8]) {
-    T t6[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-    t6[dof] = T(1);
+    T t6[6];
+    for (int k = 0; k < 6; ++k) t6[k] = k == dof ? T(1) : T(0);
     T u, p, bv[8], ibv[8];
     exp_parts(t6, u, p, bv, ibv);  // bv = the basis bivector in even-store layout
     T d[8];
