@@ -103,7 +103,7 @@ struct IslandShared {
     int nbr_start[MAXM + 1], nbr[MAXM * MAXM];
     int term_start[MAXM * MAXM + 1], terms[4 * JD];
     int inc_start[MAXM + 1], inc[2 * JD];
-    int ibcast[8];
+    int ibcast[8];                          // [0]: a non-zero Newton correction was written; [1]: seeded flag out
 };
 
 // Block-wide sum of one value per thread: a butterfly within each 64-lane wave,
@@ -196,21 +196,6 @@ struct IslandSolver {
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
         const int P = J * 12, t = tid();
-        if constexpr (!JF) {
-            // pose columns first, then velocity columns: a thread that takes a second
-            // column takes a cheap one (the mapping below measured slower for f64 lanes)
-            for (int t2 = t; t2 < J * 24; t2 += NTH) {
-                const int pose_item = t2 < J * 12;
-                const int q = pose_item ? t2 : t2 - J * 12;
-                const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
-                // the f64 lanes also deliver the wrench values
-                const bool values = ax == 0;
-                joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt,
-                             warp, s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
-                             values ? s.conn0[j][1] : nullptr);
-            }
-            return;
-        }
         int first, second;
         if (3 * P <= 2 * NTH) {
             // one pose column (tangents through the geometry) per thread; the cheaper
@@ -302,6 +287,7 @@ struct IslandSolver {
     __device__ void assemble() {
         double pf = -(half * half), vf = -half;
         double* Am = A();
+        if (tid() == 0) s.ibcast[0] = 0;  // the correction's non-zero flag (set by matvec)
         // the non-zero blocks only (neighbour lists); the others stay as the launch
         // zeroed them, which is what 0·A + 0 would write
         for (int e = tid(); e < s.nbr_start[m] * 6; e += NTH) {
@@ -566,7 +552,9 @@ struct IslandSolver {
     // ───── dv = 0·dv + Σ_j X[i,j]·rhs_j ─────
     // MFMA engine: four lanes per row, each summing every fourth block, combined by
     // two butterfly steps (all lanes of a wave take part in the shuffles). The ordered
-    // engine keeps one lane per row and the reference's block order.
+    // engine keeps one lane per row and the reference's block order. A non-zero entry
+    // sets s.ibcast[0] (cleared before the system is assembled; read after the barrier
+    // that follows).
     __device__ void matvec() {
         const double* Xm = X();
         static_assert(4 * 6 * MAXM <= NTH, "four lanes per row of the largest island");
@@ -583,7 +571,11 @@ struct IslandSolver {
                 }
             sum += __shfl_xor(sum, 1, 64);
             sum += __shfl_xor(sum, 2, 64);
-            if (live && part == 0) s.dv[row / 6][row % 6] = 0.0 * s.dv[row / 6][row % 6] + sum;
+            if (live && part == 0) {
+                const double d = 0.0 * s.dv[row / 6][row % 6] + sum;
+                s.dv[row / 6][row % 6] = d;
+                if (d != 0.0) s.ibcast[0] = 1;
+            }
             return;
         }
         for (int t = tid(); t < n; t += NTH) {
@@ -596,6 +588,7 @@ struct IslandSolver {
                 acc = acc + sum;
             }
             s.dv[i][r] = acc;
+            if (acc != 0.0) s.ibcast[0] = 1;
         }
     }
     __device__ void copy_vmid(int dst, const double (*src)[6]) {
@@ -652,11 +645,7 @@ struct IslandSolver {
             matvec();
             __syncthreads();
             // all-zero correction → hand over to subdivision
-            {
-                int nz = 0;
-                for (int t = tid(); t < m * 6; t += NTH) nz |= (s.dv[t / 6][t % 6] != 0.0);
-                if (!__syncthreads_or(nz)) break;
-            }
+            if (uni(s.ibcast[0]) == 0) break;
             // backtracking line search from the same base, α = 2^-lane for lanes 0..7,
             // evaluated LS at a time; the first lane (in order) that lowers the
             // residual is taken, as when all eight are evaluated together
