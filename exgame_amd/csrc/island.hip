@@ -373,6 +373,40 @@ struct IslandSolver {
             out[(e / n) * LD + e % n] = cell[u];
         }
     }
+    // Slab loop over NT tiles: operands of slab ks+1 are loaded while slab ks
+    // multiplies (all loads stay inside the MAXN×LDM buffers; skipped slabs are loaded
+    // and not used: loads under scalar branches measured 12 % slower, the waits no
+    // longer overlap).
+    template <int NT>
+    __device__ __forceinline__ void mfma_slabs(const double* am, const double* bm, const int* ti, const int* tj,
+                                               const unsigned* mk, d4* acc, int lr, int lk) {
+        constexpr int MAXKS = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 4;
+        double an[NT], bn[NT];
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            an[u] = am[(ti[u] * 16 + lr) * LD + lk];
+            bn[u] = bm[lk * LD + tj[u] * 16 + lr];
+        }
+#pragma unroll
+        for (int ks = 0; ks < MAXKS; ++ks) {
+            double ac[NT], bc[NT];
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                ac[u] = an[u];
+                bc[u] = bn[u];
+            }
+            if (ks + 1 < MAXKS) {
+#pragma unroll
+                for (int u = 0; u < NT; ++u) {
+                    an[u] = am[(ti[u] * 16 + lr) * LD + 4 * (ks + 1) + lk];
+                    bn[u] = bm[(4 * (ks + 1) + lk) * LD + tj[u] * 16 + lr];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                if (mk[u] & (1u << ks)) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u], acc[u], 0, 0, 0);
+        }
+    }
     // ───── MFMA form: out = a·b over 16×16 tiles, k in 4-column slabs ─────
     // Wave w computes tiles w, w+4, … of the n16×n16 product; a slab whose bit is
     // clear in kmask[tile row] is all-zero in `a` and skipped. With make_t the
@@ -389,7 +423,6 @@ struct IslandSolver {
         const int lr = lane & 15, lk = lane >> 4;
         const int T = n16 >> 4, KS = n16 >> 2;
         const unsigned dense = (1u << KS) - 1u;
-        constexpr int MAXKS = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 4;
         constexpr int MAXT = IslandShared<MAXM, MAXJ, NTH, JF>::MAXN / 16;
         constexpr int PER_WAVE = (MAXT * MAXT + NTH / 64 - 1) / (NTH / 64);
         double part = 0.0;
@@ -411,32 +444,17 @@ struct IslandSolver {
             mk[u] = uni(live ? km : 0u);
             acc[u] = d4{0.0, 0.0, 0.0, 0.0};
         }
-        // operands of slab ks+1 are loaded while slab ks multiplies (all loads stay
-        // inside the MAXN×LDM buffers; skipped slabs are loaded and not used)
-        double an[PER_WAVE], bn[PER_WAVE];
-#pragma unroll
-        for (int u = 0; u < PER_WAVE; ++u) {
-            an[u] = am[(ti[u] * 16 + lr) * LD + lk];
-            bn[u] = bm[lk * LD + tj[u] * 16 + lr];
-        }
-#pragma unroll
-        for (int ks = 0; ks < MAXKS; ++ks) {
-            double ac[PER_WAVE], bc[PER_WAVE];
-#pragma unroll
-            for (int u = 0; u < PER_WAVE; ++u) {
-                ac[u] = an[u];
-                bc[u] = bn[u];
-            }
-            if (ks + 1 < MAXKS) {
-#pragma unroll
-                for (int u = 0; u < PER_WAVE; ++u) {
-                    an[u] = am[(ti[u] * 16 + lr) * LD + 4 * (ks + 1) + lk];
-                    bn[u] = bm[(4 * (ks + 1) + lk) * LD + tj[u] * 16 + lr];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < PER_WAVE; ++u)
-                if (mk[u] & (1u << ks)) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(ac[u], bc[u], acc[u], 0, 0, 0);
+        // the slab loop for exactly this wave's tile count (a uniform branch), so that
+        // no operands are loaded for unused slots
+        if constexpr (PER_WAVE >= 3) {
+            if (ntiles >= 3) mfma_slabs<3>(am, bm, ti, tj, mk, acc, lr, lk);
+            else if (ntiles == 2) mfma_slabs<2>(am, bm, ti, tj, mk, acc, lr, lk);
+            else if (ntiles == 1) mfma_slabs<1>(am, bm, ti, tj, mk, acc, lr, lk);
+        } else if constexpr (PER_WAVE == 2) {
+            if (ntiles >= 2) mfma_slabs<2>(am, bm, ti, tj, mk, acc, lr, lk);
+            else if (ntiles == 1) mfma_slabs<1>(am, bm, ti, tj, mk, acc, lr, lk);
+        } else {
+            if (ntiles >= 1) mfma_slabs<1>(am, bm, ti, tj, mk, acc, lr, lk);
         }
         if (in_place) __syncthreads();
         // epilogue; off the diagonal of an unpadded matrix T = −R and ‖I − R‖² takes v²
@@ -484,10 +502,21 @@ struct IslandSolver {
     __device__ double refine_inverse() {  // newton.rs:255-279
         double before;
         if (mfma) {
-            // T = 2I − A·X (A block-sparse: zero slabs skipped), then X ← X·T in place
+            // T = 2I − A·X (A block-sparse: zero slabs skipped), then X ← X·T in place.
+            // ‖I − A·X‖²: wave sums into s.red before the barrier that publishes T,
+            // added in wave order after the second product (s.red's previous readers
+            // finished before the previous iteration's closing barrier; the next
+            // writers come after this one's)
             double part = mfma_product(A(), X(), R(), s.kmask, true, false, 0);
-            before = block_sum(s, part);  // its barrier also publishes T
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+            if ((tid() & 63) == 0) s.red[tid() >> 6] = part;
+            __syncthreads();
             mfma_product(X(), R(), X(), nullptr, false, true, 1);
+            double r = s.red[0];
+#pragma unroll
+            for (int w = 1; w < NTH / 64; ++w) r = r + s.red[w];
+            before = uni(r);
         } else {
             gemm(A(), X(), R(), false, true, false);
             __syncthreads();
