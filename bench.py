@@ -10,6 +10,11 @@ A bench step is one world step: gravity over all bodies, then every island's
 implicit Newton solve, re-anchoring and charge publication. The timed region runs
 K steps of a freshly uploaded world (state resident in HBM before the clock
 starts), bracketed by device synchronisation. One JSON line on stdout.
+
+Beside the headline line: the same rollout with the FP32 Jacobian and FP32 gravity
+(`mixed_precision`), and the explicit configuration the reference runs on its CPU —
+1,024 point masses, symplectic Euler, dt = 1e-3, 100 steps (`plain_gravity_explicit`,
+eager and graph-replayed, with the oracle on its first 10 steps as its CPU baseline).
 """
 from __future__ import annotations
 
@@ -73,6 +78,47 @@ def cpu_baseline(target_s: float = 10.0):
     return {"value": scene.n_bodies * n / el, "unit": "body-steps/s", "cores": threads, "kind": "port",
             "sample": f"oracle, 64 chains x 8 bodies (512 bodies), {n} warm implicit steps after 1 cold, "
                       f"{el:.1f} s on {threads} OpenMP threads"}
+
+
+def cpu_plain_gravity_explicit(n: int, steps: int, dt: float):
+    """The oracle on the first `steps` steps of the 1,024-body explicit rollout (a
+    second, bounded sample of the CPU baseline leg)."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle
+    from exgame_amd import scenes
+    scene = scenes.plain_gravity(n=n, seed=0)
+    w = pyoracle.OracleWorld(scene, integrator="symplectic_euler")
+    t0 = time.perf_counter()
+    w.step(steps, dt, 1.0)
+    el = time.perf_counter() - t0
+    return {"value": n * steps / el, "unit": "body-steps/s", "cores": pyoracle.max_threads(), "kind": "port",
+            "sample": f"oracle, the first {steps} steps of the {n}-body explicit rollout, {el:.2f} s"}
+
+
+def plain_gravity_explicit(n: int = 1024, steps: int = 100, dt: float = 1e-3):
+    """The reference's CPU-runnable configuration on the device: N point masses,
+    explicit symplectic Euler, size-core gravity, dt = 1e-3, 100 steps; stepped
+    eagerly and by replaying a captured graph of two steps."""
+    from exgame_amd import scenes
+    from exgame_amd.world import DeviceWorld
+    scene = scenes.plain_gravity(n=n, seed=0)
+    out = {"workload": f"plain_gravity_explicit_f64: {n} point masses, symplectic Euler, size-core gravity G=1, "
+                       f"dt={dt:g}, {steps} steps"}
+    for graph in (False, True):
+        w = DeviceWorld(scene, integrator="symplectic_euler", graph=graph)
+        w.step(2, dt, 1.0)   # code objects (and the graph's capture)
+        w.sync()
+        w.close()
+        w = DeviceWorld(scene, integrator="symplectic_euler", graph=graph)
+        w.sync()
+        t0 = time.perf_counter()
+        w.step(steps, dt, 1.0)
+        w.sync()
+        el = time.perf_counter() - t0
+        w.close()
+        out["graph" if graph else "eager"] = {"value": n * steps / el, "unit": "body-steps/s",
+                                              "ms_per_step": el * 1e3 / steps}
+    return out
 
 
 def measured_traffic():
@@ -200,11 +246,16 @@ def main() -> int:
                        "meets_1e-9": bool(err <= 1e-9), "gravity": gp, "jacobian": jp}
         w.close()
     line["mixed_precision"] = mixed
+    line["plain_gravity_explicit"] = plain_gravity_explicit()
     if not args.no_cpu_baseline:
         try:
             line["cpu_baseline"] = cpu_baseline()
         except Exception as e:  # the baseline is reported beside the result, never instead of it
             line["cpu_baseline"] = {"value": None, "error": str(e)}
+        try:
+            line["plain_gravity_explicit"]["cpu_baseline"] = cpu_plain_gravity_explicit(1024, 10, 1e-3)
+        except Exception as e:
+            line["plain_gravity_explicit"]["cpu_baseline"] = {"value": None, "error": str(e)}
     print(json.dumps(line), flush=True)
     return 0
 

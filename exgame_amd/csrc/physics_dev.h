@@ -423,7 +423,7 @@ __host__ __device__ inline void body_post(const S pose_mid[8], const S vel[6], c
 // composition, momentum by the midpoint rule (2·P_mid − P_n when converged, else
 // the implicit-Euler reading P_mid), back to the body frame.
 template <class S>
-__host__ __device__ inline void commit_body(S pose[8], S mom[6], const S& mass, const S t9[9], const S vs[6],
+__host__ __device__ __forceinline__ void commit_body(S pose[8], S mom[6], const S& mass, const S t9[9], const S vs[6],
                                             const S snap[6], const S& span, const S& half, bool converged) {
     S e[8], pose_mid[8], inv[8];
     twist_exp(vs, half, e);

@@ -162,6 +162,21 @@ def test_plain_gravity_explicit_matches_oracle(integrator):
     assert rel(a["momentum"][:, :3], b["momentum"][:, :3]) < 1e-10
 
 
+def test_plain_gravity_explicit_1024_bodies_matches_oracle():
+    # the explicit configuration at its full size (1,024 point masses, dt = 1e-3): the
+    # gravity-free bootstrap step and four steps under gravity, every step compared
+    scene = scenes.plain_gravity(n=1024, seed=0)
+    dev = DeviceWorld(scene, integrator="symplectic_euler")
+    orc = pyoracle.OracleWorld(scene, integrator="symplectic_euler")
+    for _ in range(5):
+        dev.step(1, 1e-3, 1.0)
+        orc.step(1, 1e-3, 1.0)
+        a, b = dev.state(), orc.state()
+        assert rel(a["position"], b["position"]) < 1e-12
+        assert rel(a["momentum"][:, :3], b["momentum"][:, :3]) < 1e-12
+    assert rel(dev.charges(), orc.front()) < 1e-12
+
+
 @pytest.mark.parametrize("steps", [1, 6])
 def test_free_bodies_under_newton_one_thread_each(steps):
     # every island a single body without joints: the thread-per-island kernel. Close
