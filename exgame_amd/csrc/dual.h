@@ -45,9 +45,36 @@ __host__ __device__ inline float value_of(float x) { return x; }
 
 __host__ __device__ inline double s_sqrt(double x) { return sqrt(x); }
 __host__ __device__ inline float s_sqrt(float x) { return sqrtf(x); }
-__host__ __device__ inline double s_sin(double x) { return sin(x); }
+// sin and cos of the half-angles of one step's rotation (|x| ≤ 1/4 in practice): the
+// Maclaurin series through x¹⁵ (sin) and x¹⁶ (cos), whose first omitted terms are
+// below 2⁻⁸⁰ of the result there, evaluated by Horner's rule; the library functions
+// (with their range reduction) beyond 1/4.
+__host__ __device__ inline double s_sin(double x) {
+    if (!(fabs(x) <= 0.25)) return sin(x);
+    const double z = x * x;
+    double p = -1.0 / 1307674368000.0;          // −1/15!
+    p = fma(p, z, 1.0 / 6227020800.0);          // 1/13!
+    p = fma(p, z, -1.0 / 39916800.0);           // −1/11!
+    p = fma(p, z, 1.0 / 362880.0);              // 1/9!
+    p = fma(p, z, -1.0 / 5040.0);               // −1/7!
+    p = fma(p, z, 1.0 / 120.0);                 // 1/5!
+    p = fma(p, z, -1.0 / 6.0);                  // −1/3!
+    return fma(x * z, p, x);
+}
+__host__ __device__ inline double s_cos(double x) {
+    if (!(fabs(x) <= 0.25)) return cos(x);
+    const double z = x * x;
+    double p = 1.0 / 20922789888000.0;          // 1/16!
+    p = fma(p, z, -1.0 / 87178291200.0);        // −1/14!
+    p = fma(p, z, 1.0 / 479001600.0);           // 1/12!
+    p = fma(p, z, -1.0 / 3628800.0);            // −1/10!
+    p = fma(p, z, 1.0 / 40320.0);               // 1/8!
+    p = fma(p, z, -1.0 / 720.0);                // −1/6!
+    p = fma(p, z, 1.0 / 24.0);                  // 1/4!
+    p = fma(p, z, -0.5);
+    return fma(z, p, 1.0);
+}
 __host__ __device__ inline float s_sin(float x) { return sinf(x); }
-__host__ __device__ inline double s_cos(double x) { return cos(x); }
 __host__ __device__ inline float s_cos(float x) { return cosf(x); }
 
 template <class T> __host__ __device__ inline Dual<T> s_sqrt(const Dual<T>& a) {
