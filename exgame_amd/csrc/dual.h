@@ -59,7 +59,7 @@ __host__ __device__ inline double s_sin(double x) {
     p = fma(p, z, -1.0 / 5040.0);               // −1/7!
     p = fma(p, z, 1.0 / 120.0);                 // 1/5!
     p = fma(p, z, -1.0 / 6.0);                  // −1/3!
-    return fma(x * z, p, x);
+    return z == 0.0 ? x : fma(x * z, p, x);     // x itself (and its sign) when x² underflows
 }
 __host__ __device__ inline double s_cos(double x) {
     if (!(fabs(x) <= 0.25)) return cos(x);

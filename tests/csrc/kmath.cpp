@@ -9,6 +9,12 @@ using namespace exg;
 extern "C" {
 
 void km_twist_exp(const double* t6, double dt, double* m8) { twist_exp(t6, dt, m8); }
+void km_sincos(const double* x, int n, double* s, double* c) {
+    for (int i = 0; i < n; ++i) {
+        s[i] = s_sin(x[i]);
+        c[i] = s_cos(x[i]);
+    }
+}
 void km_compose(const double* a8, const double* b8, double* out8) { compose(a8, b8, out8); }
 void km_point(const double* m8, const double* p3, double* out4) {
     double w, q[3];

@@ -46,6 +46,21 @@ def rel(a, b):
     return np.abs(np.asarray(a) - np.asarray(b)).max() / max(1.0, np.abs(np.asarray(b)).max())
 
 
+def test_small_angle_sin_cos_within_one_ulp(km):
+    # the kernels' sin/cos: a series for |x| <= 1/4, the library beyond; against the
+    # host libm through numpy (correctly rounded to within an ulp) and the exact
+    # special values
+    x = np.concatenate([RNG.uniform(-0.25, 0.25, 20000), np.ldexp(RNG.uniform(-1, 1, 2000), -RNG.integers(3, 60, 2000)),
+                        [0.25, -0.25, np.nextafter(0.25, 1), 0.3, 1.0, 3.0, -7.5, 0.0, -0.0, 1e-300]])
+    s = np.zeros_like(x)
+    c = np.zeros_like(x)
+    km.km_sincos(x.ctypes.data_as(_dp), len(x), s.ctypes.data_as(_dp), c.ctypes.data_as(_dp))
+    ulp_s = np.abs(s - np.sin(x)) / np.spacing(np.abs(np.sin(x)) + 1e-320)
+    ulp_c = np.abs(c - np.cos(x)) / np.spacing(np.abs(np.cos(x)))
+    assert ulp_s.max() <= 1.0 and ulp_c.max() <= 1.0, (ulp_s.max(), ulp_c.max())
+    assert s[-3] == 0.0 and np.signbit(s[-2]) and c[-3] == 1.0 and s[-1] == 1e-300
+
+
 def test_motor_ops(km):
     for _ in range(10):
         t = RNG.normal(size=6)
