@@ -161,7 +161,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (worlds shard by island with no "
-              f"exchange, gravity by target rows with one all-gather; see DESIGN.md)", file=sys.stderr)
+              f"exchange, gravity by target rows with one all-gather; this build runs on one GPU)", file=sys.stderr)
         return 2
 
     from exgame_amd import scenes, _lib

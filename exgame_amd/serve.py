@@ -11,8 +11,8 @@ and is retired, freeing room for the next. Because an arena world's trajectory d
 not depend on its neighbours, a request's result is the same alone and in the mix.
 
 The reference runs rollouts one Driver at a time (crates/newton/src/driver.rs);
-this loop is the serving layer that BASELINE's serving form asks for on top of the
-batched step.
+this loop is a serving layer (Poisson arrivals, admission by HBM footprint,
+continuous batching, p50/p99 latency) on top of the batched step.
 """
 from __future__ import annotations
 
