@@ -583,7 +583,8 @@ struct IslandSolver {
     // two butterfly steps (all lanes of a wave take part in the shuffles). The ordered
     // engine keeps one lane per row and the reference's block order. A non-zero entry
     // sets s.ibcast[0] (cleared before the system is assembled; read after the barrier
-    // that follows).
+    // that follows); each entry's writer also copies the iterate into the line
+    // search's base.
     __device__ void matvec() {
         const double* Xm = X();
         static_assert(4 * 6 * MAXM <= NTH, "four lanes per row of the largest island");
@@ -603,6 +604,7 @@ struct IslandSolver {
             if (live && part == 0) {
                 const double d = 0.0 * s.dv[row / 6][row % 6] + sum;
                 s.dv[row / 6][row % 6] = d;
+                s.base[row / 6][row % 6] = s.vmid0[row / 6][row % 6];
                 if (d != 0.0) s.ibcast[0] = 1;
             }
             return;
@@ -617,6 +619,7 @@ struct IslandSolver {
                 acc = acc + sum;
             }
             s.dv[i][r] = acc;
+            s.base[i][r] = s.vmid0[i][r];
             if (acc != 0.0) s.ibcast[0] = 1;
         }
     }
@@ -678,8 +681,8 @@ struct IslandSolver {
             // backtracking line search from the same base, α = 2^-lane for lanes 0..7,
             // evaluated LS at a time; the first lane (in order) that lowers the
             // residual is taken, as when all eight are evaluated together
-            for (int t = tid(); t < m * 6; t += NTH) s.base[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
-            __syncthreads();
+            // (the line search's base, a copy of the iterate, was written beside dv by
+            // the matvec and published by its barrier)
             int accepted = -1, accepted_slot = -1;
             double tnorm = 0.0;
             for (int round = 0; round < LANES / LS && accepted < 0; ++round) {
@@ -771,8 +774,15 @@ struct IslandSolver {
     __device__ void publish_x(bool& accepted_seeded) {
         const double* Xm = X();
         double* dst = a.xacc + xoff;
-        for (int e = tid(); e < n * n; e += NTH) dst[e] = Xm[(e / n) * LD + e % n];
-        double nx = reduce_sq(Xm, false);
+        // the copy and ‖X‖² in one pass (each thread's terms in reduce_sq's order)
+        double part = 0.0;
+        for (int e = tid(); e < n * n; e += NTH) {
+            const double v = Xm[(e / n) * LD + e % n];
+            dst[e] = v;
+            const double x = 0.0 - v;
+            part = part + x * x;
+        }
+        const double nx = block_sum(s, part);
         const double huge = 1000000000.0;
         if (nx < huge * huge) accepted_seeded = seeded;
         else { seeded = false; accepted_seeded = false; }
