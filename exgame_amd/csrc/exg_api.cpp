@@ -96,7 +96,7 @@ struct exg_world {
     int front = 0;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
-    bool timing_split = false;   // record the gravity/island split event inside a step
+    hipEvent_t* step_ev = nullptr;   // this step's four timing events (gravity start/stop, island start/stop)
     bool use_graph = false;
     bool graph_ms_valid = false;
     int settings_epoch = 0;      // bumped when a setting baked into kernel arguments changes
@@ -315,14 +315,20 @@ int exg_world_set_ns_engine(exg_world* w, int engine) {
 static int enqueue_world_step(exg_world* w, double dt, double warp) {
     double* front = w->charges[w->front].p;
     double* back = w->charges[1 - w->front].p;
+    // timing events are recorded by the dispatches themselves (no marker packets)
+    hipEvent_t* ev = w->step_ev;
+    hipEvent_t g0 = ev ? ev[0] : nullptr, g1 = ev ? ev[1] : nullptr;
     if (w->gravity) {
         exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p, (int)w->grav_tiles.n,
-                            w->grav_prec, w->grav_partial.p, w->ext.p, w->stream);
+                            w->grav_prec, w->grav_partial.p, w->ext.p, w->stream, g0, g1);
     } else {
-        exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream);
+        exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream, g0, g1);
     }
     HIP_TRY(hipGetLastError());
-    if (w->timing_split) HIP_TRY(hipEventRecord(w->ev[3 * w->ev_used + 1], w->stream));
+    if (ev && !(w->gravity ? w->grav_tiles.n > 0 : w->B > 0)) {  // nothing launched: mark the (empty) phase
+        HIP_TRY(hipEventRecord(g0, w->stream));
+        HIP_TRY(hipEventRecord(g1, w->stream));
+    }
     exg::IslandArgs a{};
     a.pose = w->pose.p;
     a.mom = w->mom.p;
@@ -354,16 +360,27 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.ns_engine = w->ns_engine;
     a.jac_f32 = w->jac_prec == EXG_PREC_F32_COMPENSATED;
     a.n_islands = w->I;
+    hipEvent_t i0 = ev ? ev[2] : nullptr, i1 = ev ? ev[3] : nullptr;
     if (w->integrator == EXG_NEWTON) {
+        int first = -1, last = -1;
+        for (int c = 0; c < exg::ISLAND_CLASSES; ++c)
+            if (w->class_n[c]) {
+                if (first < 0) first = c;
+                last = c;
+            }
         for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
             if (!w->class_n[c]) continue;
             a.island_ids = w->class_ids[c].p;
             a.n_islands = w->class_n[c];
-            exg::launch_island_class(a, c, w->stream);
+            exg::launch_island_class(a, c, w->stream, c == first ? i0 : nullptr, c == last ? i1 : nullptr);
+        }
+        if (ev && first < 0) {
+            HIP_TRY(hipEventRecord(i0, w->stream));
+            HIP_TRY(hipEventRecord(i1, w->stream));
         }
     } else {
         a.island_ids = nullptr;
-        exg::launch_explicit_step(a, w->integrator, w->stream);
+        exg::launch_explicit_step(a, w->integrator, w->stream, i0, i1);
     }
     HIP_TRY(hipGetLastError());
     w->front = 1 - w->front;
@@ -381,14 +398,12 @@ static int world_graph(exg_world* w, double dt, double warp, hipGraphExec_t* out
     if (g.exec) (void)hipGraphExecDestroy(g.exec);
     g.exec = nullptr;
     const int front0 = w->front;
-    const bool split = w->timing_split;
-    w->timing_split = false;
+    w->step_ev = nullptr;
     HIP_TRY(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_world_step(w, dt, warp);
     if (rc == EXG_OK) rc = enqueue_world_step(w, dt, warp);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(w->stream, &graph);
-    w->timing_split = split;
     w->front = front0;
     if (rc != EXG_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -422,27 +437,26 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
         HIP_TRY(hipEventRecord(w->ev[0], w->stream));
         for (int k = 0; k + 1 < n_steps; k += 2) HIP_TRY(hipGraphLaunch(exec, w->stream));
         if (n_steps % 2) {
-            w->timing_split = false;
+            w->step_ev = nullptr;
             if (int rc = enqueue_world_step(w, dt, warp)) return rc;
         }
         HIP_TRY(hipEventRecord(w->ev[1], w->stream));
         w->graph_ms_valid = true;
         return EXG_OK;
     }
-    size_t need = (size_t)n_steps * 3;
+    size_t need = (size_t)n_steps * 4;
     while (w->ev.size() < need) {
         hipEvent_t e;
         HIP_TRY(hipEventCreate(&e));
         w->ev.push_back(e);
     }
-    w->timing_split = true;
     for (int s = 0; s < n_steps; ++s) {
-        HIP_TRY(hipEventRecord(w->ev[3 * s], w->stream));
-        if (int rc = enqueue_world_step(w, dt, warp)) return rc;
-        HIP_TRY(hipEventRecord(w->ev[3 * s + 2], w->stream));
+        w->step_ev = &w->ev[4 * (size_t)s];
+        const int rc = enqueue_world_step(w, dt, warp);
+        w->step_ev = nullptr;
+        if (rc) return rc;
         w->ev_used = s + 1;
     }
-    w->timing_split = false;
     return EXG_OK;
 }
 
@@ -471,8 +485,8 @@ int exg_world_last_timing(exg_world* w, double* out2) {
     double g = 0.0, s = 0.0;
     for (int k = 0; k < w->ev_used; ++k) {
         float a = 0.f, b = 0.f;
-        HIP_TRY(hipEventElapsedTime(&a, w->ev[3 * k], w->ev[3 * k + 1]));
-        HIP_TRY(hipEventElapsedTime(&b, w->ev[3 * k + 1], w->ev[3 * k + 2]));
+        HIP_TRY(hipEventElapsedTime(&a, w->ev[4 * k], w->ev[4 * k + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, w->ev[4 * k + 2], w->ev[4 * k + 3]));
         g += a;
         s += b;
     }

@@ -109,9 +109,9 @@ __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs 
     }
 }
 
-void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream) {
+void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
     int blocks = (a.n_islands + EXPL_THREADS - 1) / EXPL_THREADS;
-    hipLaunchKernelGGL(explicit_step_kernel, dim3(blocks), dim3(EXPL_THREADS), 0, stream, a, integrator);
+    launch_k(explicit_step_kernel, dim3(blocks), dim3(EXPL_THREADS), stream, start, stop, a, integrator);
 }
 
 }  // namespace exg

@@ -314,27 +314,29 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
 }
 
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
-                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream) {
+                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream, hipEvent_t start,
+                    hipEvent_t stop) {
     if (n_tiles <= 0) return;
     // tiles without entries (worlds without gravity) still get their zero wrench from
     // the reduce, also when no world of the launch has gravity
     if (n_entries <= 0) {
-        hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), 0, stream, charges, tiles, partial,
-                           ext);
+        launch_k(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), stream, start, stop, charges, tiles,
+                 (const double*)partial, ext);
         return;
     }
     if (precision == 1)
-        hipLaunchKernelGGL(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), 0, stream, charges,
-                           entries, partial);
+        launch_k(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
+                 entries, partial);
     else
-        hipLaunchKernelGGL(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), 0, stream, charges,
-                           entries, partial);
-    hipLaunchKernelGGL(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), 0, stream, charges, tiles, partial, ext);
+        launch_k(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
+                 entries, partial);
+    launch_k(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), stream, nullptr, stop, charges, tiles,
+             (const double*)partial, ext);
 }
 
-void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream) {
+void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(zero_wrench_kernel, dim3((n * 6 + ZERO_T - 1) / ZERO_T), dim3(ZERO_T), 0, stream, ext, b0, n);
+    launch_k(zero_wrench_kernel, dim3((n * 6 + ZERO_T - 1) / ZERO_T), dim3(ZERO_T), stream, start, stop, ext, b0, n);
 }
 
 }  // namespace exg

@@ -1,6 +1,7 @@
 // Launch interface of the device kernels of the dynamics hot path.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cstdint>
 #include <vector>
 
@@ -79,12 +80,25 @@ void island_class_shape(int cls, int& waves, int& per_wave);
 // build_island_topology for island i of a world: bodies [b0, b0 + m), joints with
 // global body indices a/b; routes by island_class.
 int island_topology_of(int m, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t);
-void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream);
+// One kernel launch; with a start or stop event the dispatch itself records them
+// (hipExtLaunchKernelGGL), otherwise a plain launch (which is also what stream
+// capture records).
+template <typename F, typename... A>
+inline void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t stream, hipEvent_t start, hipEvent_t stop, A... args) {
+    if (start || stop) hipExtLaunchKernelGGL(kernel, grid, block, 0, stream, start, stop, 0, args...);
+    else hipLaunchKernelGGL(kernel, grid, block, 0, stream, args...);
+}
+
+// Launchers take optional start/stop events that the dispatch itself records
+// (hipExtLaunchKernelGGL: no marker packets between kernels); null records nothing.
+void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start = nullptr,
+                         hipEvent_t stop = nullptr);
 // Single-body islands without joints, one thread per island (singles.hip).
-void launch_singles(const IslandArgs& a, hipStream_t stream);
+void launch_singles(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.
-void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream);
+void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream, hipEvent_t start = nullptr,
+                          hipEvent_t stop = nullptr);
 
 // Pairwise gravity over a work table (gravity.hip): entries pair a tile of ≤ 64
 // targets with a chunk of sources of the same world; tiles list their entries in
@@ -106,7 +120,9 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
                                int chunk_len);
 // precision 0: f64 pairs; 1: f32 pairs with compensated accumulation
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
-                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream);
-void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream);
+                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream, hipEvent_t start = nullptr,
+                    hipEvent_t stop = nullptr);
+void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream, hipEvent_t start = nullptr,
+                        hipEvent_t stop = nullptr);
 
 }  // namespace exg

@@ -1035,23 +1035,24 @@ void island_class_shape(int cls, int& waves, int& per_wave) {
     }
 }
 
-void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream) {
+void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
     if (a.n_islands <= 0) return;
+    const dim3 grid(a.n_islands);
     switch (cls) {
         case ISLAND_CLASS_M1:
-            launch_singles(a, stream);
+            launch_singles(a, stream, start, stop);
             break;
         case ISLAND_CLASS_M4J8:
-            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m4j8_jf32, dim3(a.n_islands), dim3(128), 0, stream, a);
-            else hipLaunchKernelGGL(island_step_kernel_m4j8, dim3(a.n_islands), dim3(128), 0, stream, a);
+            if (a.jac_f32) launch_k(island_step_kernel_m4j8_jf32, grid, dim3(128), stream, start, stop, a);
+            else launch_k(island_step_kernel_m4j8, grid, dim3(128), stream, start, stop, a);
             break;
         case ISLAND_CLASS_M8J14:
-            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m8j14_jf32, dim3(a.n_islands), dim3(256), 0, stream, a);
-            else hipLaunchKernelGGL(island_step_kernel_m8j14, dim3(a.n_islands), dim3(256), 0, stream, a);
+            if (a.jac_f32) launch_k(island_step_kernel_m8j14_jf32, grid, dim3(256), stream, start, stop, a);
+            else launch_k(island_step_kernel_m8j14, grid, dim3(256), stream, start, stop, a);
             break;
         default:
-            if (a.jac_f32) hipLaunchKernelGGL(island_step_kernel_m8j16_jf32, dim3(a.n_islands), dim3(256), 0, stream, a);
-            else hipLaunchKernelGGL(island_step_kernel_m8j16, dim3(a.n_islands), dim3(256), 0, stream, a);
+            if (a.jac_f32) launch_k(island_step_kernel_m8j16_jf32, grid, dim3(256), stream, start, stop, a);
+            else launch_k(island_step_kernel_m8j16, grid, dim3(256), stream, start, stop, a);
             break;
     }
 }

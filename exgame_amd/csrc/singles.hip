@@ -395,10 +395,10 @@ __global__ __launch_bounds__(SB) void singles_step_kernel(IslandArgs a) {
 
 }  // namespace
 
-void launch_singles(const IslandArgs& a, hipStream_t stream) {
+void launch_singles(const IslandArgs& a, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
     if (a.n_islands <= 0) return;
     const long long lanes = (long long)a.n_islands * SG;
-    hipLaunchKernelGGL(singles_step_kernel, dim3((unsigned)((lanes + SB - 1) / SB)), dim3(SB), 0, stream, a);
+    launch_k(singles_step_kernel, dim3((unsigned)((lanes + SB - 1) / SB)), dim3(SB), stream, start, stop, a);
 }
 
 }  // namespace exg
