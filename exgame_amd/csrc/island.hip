@@ -101,6 +101,7 @@ struct IslandShared {
     int jkind[JD], ja[JD], jb[JD];
     int jorder[JD];                         // joints grouped by kind (thread mapping only)
     int nbr_start[MAXM + 1], nbr[MAXM * MAXM];
+    unsigned char nbr_row[MAXM * MAXM];     // block row of each neighbour-list entry
     int term_start[MAXM * MAXM + 1], terms[4 * JD];
     int inc_start[MAXM + 1], inc[2 * JD];
     int ibcast[8];                          // [0]: a non-zero Newton correction was written; [1]: seeded flag out
@@ -292,8 +293,7 @@ struct IslandSolver {
         // zeroed them, which is what 0·A + 0 would write
         for (int e = tid(); e < s.nbr_start[m] * 6; e += NTH) {
             const int q = e / 6, r = e - 6 * q;
-            int i = 0;
-            while (s.nbr_start[i + 1] <= q) ++i;
+            const int i = s.nbr_row[q];
             const int j = s.nbr[q], b = i * m + j;
             const int q0 = s.term_start[b], q1 = s.term_start[b + 1];
             const double dg = (i == j) ? 1.0 : 0.0;
@@ -319,6 +319,8 @@ struct IslandSolver {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) acc[c] = acc[c] + ((double)hi[c] + (double)lo[c]);
             } else {
+                // two terms' loads in flight at a time; the sums keep the term order
+#pragma unroll 2
                 for (int q = q0; q < q1; ++q) {
                     const int code = s.terms[q];
                     const int jn = code >> 2, re = (code >> 1) & 1, ce = code & 1;
@@ -905,6 +907,13 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         }
     }
     __syncthreads();
+    // the block row of every neighbour-list entry (read by the assembly; published by
+    // the barrier that follows the world momenta)
+    if ((int)threadIdx.x < s.nbr_start[m]) {
+        int i = 0;
+        while (s.nbr_start[i + 1] <= (int)threadIdx.x) ++i;
+        s.nbr_row[threadIdx.x] = (unsigned char)i;
+    }
 
     IslandSolver<MAXM, MAXJ, NTH, JF> sv{s, a};
     sv.m = m;
