@@ -162,6 +162,35 @@ def test_plain_gravity_explicit_matches_oracle(integrator):
     assert rel(a["momentum"][:, :3], b["momentum"][:, :3]) < 1e-10
 
 
+def test_bench_world_conserves_momentum_at_full_size():
+    # the 4,096-body chains world at its full size, where the oracle is too slow to
+    # run alongside: gravity pairs, axial springs along their lines and torsional
+    # torques are equal and opposite, so the implicit steps keep the total linear and
+    # angular momentum up to rounding and the Newton tolerance
+    from exgame_amd import pga
+    scene = scenes.jointed_chains(n_chains=512, m=8, seed=0)
+    dev = DeviceWorld(scene, tol2=1e-20)
+
+    def totals():
+        st = dev.state()
+        isl = np.repeat(np.arange(scene.n_islands), np.diff(scene.island_start))
+        wm = pga.world_momentum(st["pose"], st["momentum"])
+        tau = wm[:, 3:] + np.cross(st["origin"][isl], wm[:, :3])
+        r = st["position"]
+        scale_l = np.abs(wm[:, :3]).sum()
+        scale_a = np.abs(tau).sum() + np.abs(np.cross(r, wm[:, :3])).sum()
+        return wm[:, :3].sum(0), tau.sum(0), scale_l, scale_a
+
+    dev.step(1, 1e-2, 1.0)   # the first step has no gravity (bootstrap); start after it
+    l0, a0, sl, sa = totals()
+    dev.step(20, 1e-2, 1.0)
+    l1, a1, _, _ = totals()
+    assert np.abs(l1 - l0).max() < 1e-11 * sl, (l1 - l0, sl)
+    assert np.abs(a1 - a0).max() < 1e-9 * sa, (a1 - a0, sa)
+    st = dev.stats()
+    assert st[:, 0].min() >= 21 and np.all(st[:, 4] == 0)   # every island stepped, no rejected line search
+
+
 def test_plain_gravity_explicit_1024_bodies_matches_oracle():
     # the explicit configuration at its full size (1,024 point masses, dt = 1e-3): the
     # gravity-free bootstrap step and four steps under gravity, every step compared
