@@ -95,7 +95,9 @@ struct IslandShared {
     double pose[MAXM][8], mom[MAXM][6], massv[MAXM], tens[MAXM][9], ext[MAXM][6];
     double snap[MAXM][6], dv[MAXM][6], base[MAXM][6], best[MAXM][6];
     double jparams[JD][10];
-    double red[NTH / 64];                    // wave partial sums of block_sum
+    double red[NTH / 64];                    // wave partial sums: block_sum, and ‖I − A·X‖² inside
+                                             // refine_inverse (written before the barrier that
+                                             // publishes T, read before the iteration's last one)
     double seedsum[2 * MAXM];               // seed_transpose: block-row and block-column |A| sums
     double bcast[8];
     int jkind[JD], ja[JD], jb[JD];
