@@ -29,6 +29,21 @@ def test_graph_replays_equal_eager_steps(precision):
     assert t["gravity_ms"] == -1.0 and t["island_ms"] > 0.0
 
 
+@pytest.mark.parametrize("integrator", ["symplectic_euler", "explicit_euler"])
+def test_explicit_graph_replays_equal_eager_steps(integrator):
+    # the explicit configuration as the bench replays it (1,024 point masses, dt = 1e-3)
+    scene = scenes.plain_gravity(n=1024, seed=0)
+    eager = DeviceWorld(scene, integrator=integrator)
+    graph = DeviceWorld(scene, integrator=integrator, graph=True)
+    for n in (2, 7, 10, 1, 4):
+        eager.step(n, 1e-3, 1.0)
+        graph.step(n, 1e-3, 1.0)
+        a, b = eager.state(), graph.state()
+        for k in ("pose", "momentum", "origin"):
+            assert np.array_equal(a[k], b[k]), (n, k)
+        assert np.array_equal(eager.charges(), graph.charges())
+
+
 def test_arena_graph_replays_equal_eager_steps_across_admit_and_retire():
     # worlds admitted and retired between replays (the graph is re-captured for the new
     # set of worlds); every state must equal the eager arena's bit for bit
