@@ -9,8 +9,34 @@
 
 namespace exg {
 
-// `v` as held by lane j of this lane's aligned group of eight
-__device__ __forceinline__ double group8(double v, int j) { return __shfl(v, j, 8); }
+// `v` as held by lane j of this lane's aligned group of eight. With j known at compile
+// time (the unrolled row loops) a ds_swizzle in bitmask mode broadcasts it — lane
+// ((lane & 0x18) | j) within each 32-lane half, no address to compute — otherwise a
+// general shuffle.
+template <int J>
+__device__ __forceinline__ double group8_const(double v) {
+    constexpr int pattern = 0x18 | (J << 5);   // and_mask 0b11000, or_mask J, xor_mask 0
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(bits & 0xffffffffll), pattern);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(bits >> 32), pattern);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double group8(double v, int j) {
+    if (__builtin_constant_p(j)) {
+        switch (j) {
+            case 0: return group8_const<0>(v);
+            case 1: return group8_const<1>(v);
+            case 2: return group8_const<2>(v);
+            case 3: return group8_const<3>(v);
+            case 4: return group8_const<4>(v);
+            case 5: return group8_const<5>(v);
+            case 6: return group8_const<6>(v);
+            case 7: return group8_const<7>(v);
+            default: break;
+        }
+    }
+    return __shfl(v, j, 8);
+}
 
 // entry `row` of six values, by selects on the values (passed by value, so the
 // selects cannot become a dynamically indexed load from private memory)
