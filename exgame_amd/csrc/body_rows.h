@@ -21,6 +21,28 @@ __device__ __forceinline__ double group8_const(double v) {
     const int hi = __builtin_amdgcn_ds_swizzle((int)(bits >> 32), pattern);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+// `v` from the lane whose index differs from this lane's by `Off` (a power of two
+// below 32): a ds_swizzle in bitmask mode (xor_mask Off) on the two halves.
+template <int Off>
+__device__ __forceinline__ double xor_lane(double v) {
+    static_assert(Off > 0 && Off < 32 && (Off & (Off - 1)) == 0, "xor offset within a 32-lane half");
+    constexpr int pattern = 0x1f | (Off << 10);   // and_mask 0b11111, or_mask 0, xor_mask Off
+    const long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_ds_swizzle((int)(bits & 0xffffffffll), pattern);
+    const int hi = __builtin_amdgcn_ds_swizzle((int)(bits >> 32), pattern);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// Butterfly sum over the 64 lanes of a wave, every lane ending with the same value
+// (partners at distance 32, 16, …, 1 in that order).
+__device__ __forceinline__ double wave_sum(double v) {
+    v += __shfl_xor(v, 32, 64);
+    v += xor_lane<16>(v);
+    v += xor_lane<8>(v);
+    v += xor_lane<4>(v);
+    v += xor_lane<2>(v);
+    v += xor_lane<1>(v);
+    return v;
+}
 __device__ __forceinline__ double group8(double v, int j) {
     if (__builtin_constant_p(j)) {
         switch (j) {

@@ -115,8 +115,7 @@ struct IslandShared {
 // leading barrier protects s.red from the previous call's readers.
 template <int MAXM, int MAXJ, int NTH, bool JF>
 __device__ double block_sum(IslandShared<MAXM, MAXJ, NTH, JF>& s, double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) s.red[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -512,8 +511,7 @@ struct IslandSolver {
             // finished before the previous iteration's closing barrier; the next
             // writers come after this one's)
             double part = mfma_product(A(), X(), R(), s.kmask, true, false, 0);
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+            part = wave_sum(part);
             if ((tid() & 63) == 0) s.red[tid() >> 6] = part;
             __syncthreads();
             mfma_product(X(), R(), X(), nullptr, false, true, 1);
@@ -603,8 +601,8 @@ struct IslandSolver {
                     for (int c = 0; c < 6; ++c) sb = sb + xb[c] * s.rhs0[j][c];
                     sum = sum + sb;
                 }
-            sum += __shfl_xor(sum, 1, 64);
-            sum += __shfl_xor(sum, 2, 64);
+            sum += xor_lane<1>(sum);
+            sum += xor_lane<2>(sum);
             if (live && part == 0) {
                 const double d = 0.0 * s.dv[row / 6][row % 6] + sum;
                 s.dv[row / 6][row % 6] = d;
