@@ -275,12 +275,30 @@ struct IslandSolver {
             r1 = ang2 / sc[1];
         }
         {   // every lane of every wave runs the shuffles
-            const int base = (t & 63) & ~(LSTR - 1);
             double worst = 0.0;
-            for (int q = 0; q < m; ++q) {
-                const double a0 = __shfl(r0, base + q * 8, 64), a1 = __shfl(r1, base + q * 8, 64);
-                worst = max2(worst, a0);
-                worst = max2(worst, a1);
+            if constexpr (LSTR == 64) {
+                // one candidate per wave: its bodies' ratios are read from fixed lanes
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < m) {
+                        const double a0 = __longlong_as_double(
+                            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
+                                             (int)(__double_as_longlong(r0) >> 32), q * 8) << 32) |
+                                        (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(r0), q * 8)));
+                        const double a1 = __longlong_as_double(
+                            (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
+                                             (int)(__double_as_longlong(r1) >> 32), q * 8) << 32) |
+                                        (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(r1), q * 8)));
+                        worst = max2(worst, a0);
+                        worst = max2(worst, a1);
+                    }
+            } else {
+                const int base = (t & 63) & ~(LSTR - 1);
+                for (int q = 0; q < m; ++q) {
+                    const double a0 = __shfl(r0, base + q * 8, 64), a1 = __shfl(r1, base + q * 8, 64);
+                    worst = max2(worst, a0);
+                    worst = max2(worst, a1);
+                }
             }
             if ((t % LSTR) == 0 && ll < nl) s.wn[l0 + ll] = worst;
         }
