@@ -280,7 +280,7 @@ int exg_world_destroy(exg_world* w) {
 
 int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm, int ns_seed) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_solver: null world");
-    if (integrator < EXG_EXPLICIT_EULER || integrator > EXG_NEWTON)
+    if (integrator < EXG_EXPLICIT_EULER || integrator > EXG_LIE_EULER)
         return fail(EXG_ERR_INVALID, "exg_world_set_solver: unknown integrator");
     if (ns_warm < 0 || ns_seed < 1) return fail(EXG_ERR_INVALID, "exg_world_set_solver: bad Newton–Schulz budgets");
     w->integrator = integrator;

@@ -1,10 +1,12 @@
 // Explicit integrators through the reference's bridge, one thread per island:
 // joints evaluated at the current pose and world velocity (retraction 0, value
 // only), gathered onto the external wrench, pulled back to the body frame by the
-// coadjoint transport, then SymplecticEuler or ExplicitEuler per body.
+// coadjoint transport, then SymplecticEuler, ExplicitEuler or LieEuler per body.
 //   bridge::explicit_step_all            crates/newton/src/integrator/implicit/bridge.rs:19-63
 //   SymplecticEuler::step                crates/newton/src/integrator/explicit/symplectic_euler.rs:26-39
 //   ExplicitEuler::step                  crates/newton/src/integrator/explicit/explicit_euler.rs:22-34
+//   LieEuler::step                       crates/newton/src/integrator/explicit/lie_euler.rs:26-56
+//   Motor::rotation_part / normalized    crates/clifford/src/pga3/motor.rs:231-249
 //   RigidBody::gyroscopic                crates/newton/src/body.rs:186-191
 // followed by the same re-anchoring and charge publication as the implicit path.
 #include <hip/hip_runtime.h>
@@ -17,8 +19,33 @@ constexpr int EXPL_THREADS = 64;
 constexpr int EXPL_MAXM = 8;
 constexpr int EXPL_MAXJ = 16;
 
+// The rotational part of a motor: its scalar and Euclidean-bivector components
+// (1, e12, e13, e23), divided by √(s² + b12² + b13² + b23²) = √⟨M·M̃⟩₀ unless that
+// is effectively zero.
+__device__ void rotation_part(const double m[8], double r[8]) {
+    for (int k = 0; k < 4; ++k) r[k] = m[k];
+    for (int k = 4; k < 8; ++k) r[k] = 0.0;
+    const double n2 = ((m[0] * m[0] + m[1] * m[1]) + m[2] * m[2]) + m[3] * m[3];
+    if (effective_zero(n2)) return;
+    const double inv = 1.0 / sqrt(n2);
+    for (int k = 0; k < 4; ++k) r[k] = r[k] * inv;
+}
+
 __device__ void explicit_body(double pose[8], double mom[6], double mass, const double t9[9], const double w_body[6],
                               double dt, int integrator) {
+    if (integrator == 3) {  // Lie: momentum by the wrench, then conjugated by the pose increment's rotation
+        for (int k = 0; k < 6; ++k) mom[k] = mom[k] + w_body[k] * dt;
+        double vn[6], delta[8], drot[8], dinv[8], pm[6], np[8];
+        inertia_apply_inverse(mass, t9, mom, vn);
+        twist_exp(vn, dt, delta);
+        rotation_part(delta, drot);
+        motor_rev(drot, dinv);
+        transport_co(mom, dinv, pm);
+        for (int k = 0; k < 6; ++k) mom[k] = pm[k];
+        compose(delta, pose, np);
+        for (int k = 0; k < 8; ++k) pose[k] = np[k];
+        return;
+    }
     double v[6], gyro[6], cb[6];
     inertia_apply_inverse(mass, t9, mom, v);
     coad_bracket(mom, v, cb);

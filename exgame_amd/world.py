@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib, pga
 
-INTEGRATORS = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2}
+INTEGRATORS = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2, "lie_euler": 3}
 # Newton–Schulz products: FP64 MFMA tiles, or scalar FMAs in the reference's block order
 NS_ENGINES = {"mfma": 0, "valu_ordered": 1}
 # f64 throughout, or FP32 arithmetic with two-sum compensated accumulation

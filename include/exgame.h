@@ -51,11 +51,14 @@ enum exg_joint_kind {
     EXG_TORSIONAL_DAMPER_WARPED = 3
 };
 
-/* Integrators of a world (crates/newton/src/integrator/implicit/mod.rs:20-60). */
+/* Integrators of a world: the variants of ImplicitIntegrator
+ * (crates/newton/src/integrator/implicit/mod.rs:43-48); the explicit ones run through
+ * the bridge (bridge.rs:19-63). */
 enum exg_integrator {
     EXG_EXPLICIT_EULER = 0,
     EXG_SYMPLECTIC_EULER = 1,
-    EXG_NEWTON = 2
+    EXG_NEWTON = 2,
+    EXG_LIE_EULER = 3      /* crates/newton/src/integrator/explicit/lie_euler.rs:26-56 */
 };
 
 /* Device description — the analogue of rembrandt's GpuAccelerator::new + gpu_in_flight

@@ -264,6 +264,20 @@ void orc_ns_solve(int m, const double* a, const double* rhs6m, int iters, double
         for (int r = 0; r < 6; ++r) for (int c = 0; c < 6; ++c) x_out[b * 36 + r * 6 + c] = isl.X[b][r][c];
 }
 
+// The spans of island k's last step, 4 ints each: depth, converged, committed,
+// signal (NO_SIGNAL when none was read). Returns the number of spans (≤ max kept).
+int orc_world_span_trace(void* h, int k, int32_t* out4, int max) {
+    World& w = static_cast<OrcWorld*>(h)->w;
+    const auto& tr = w.islands[k].trace;
+    for (int i = 0; i < (int)tr.size() && i < max; ++i) {
+        out4[4 * i] = tr[i].depth;
+        out4[4 * i + 1] = tr[i].converged;
+        out4[4 * i + 2] = tr[i].committed;
+        out4[4 * i + 3] = tr[i].signal;
+    }
+    return (int)tr.size();
+}
+
 void orc_world_set_uniform(void* h, double ax, double ay, double az) {
     World& w = static_cast<OrcWorld*>(h)->w;
     w.uniform[0] = ax; w.uniform[1] = ay; w.uniform[2] = az;

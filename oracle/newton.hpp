@@ -44,6 +44,13 @@ struct SolveStats {
     int unconverged_commits = 0;
 };
 
+// One attempted span of solve_step: its depth, whether Newton converged, whether the
+// span was committed, and the stiffness signal ⌊log₂(‖A‖²/‖𝕀‖²)⌋ read after it (for a
+// rejected span: the deepening input; for a committed one: the growth input), or
+// NO_SIGNAL when none was read.
+constexpr int NO_SIGNAL = -100000;
+struct SpanEvent { int depth, converged, committed, signal; };
+
 struct Island {
     std::vector<Body> bodies;   // every body is dynamic; index = row of the system
     std::vector<Joint> joints;
@@ -56,6 +63,7 @@ struct Island {
     std::vector<std::vector<BlockTerm>> block_terms;
     bool seeded = false, accepted_seeded = false;
     SolveStats stats;
+    std::vector<SpanEvent> trace;   // the spans of the last step
 };
 
 struct Lane {
@@ -425,6 +433,7 @@ struct NewtonCtx {
     void solve_step() {
         uint64_t remaining = 1ull << MAX_DEPTH;
         int depth = 0;
+        isl.trace.clear();
         while (remaining > 0) {
             uint64_t units = 1ull << (MAX_DEPTH - depth);
             while (units > remaining) { depth += 1; units >>= 1; }
@@ -442,7 +451,9 @@ struct NewtonCtx {
                 isl.X = isl.Xacc;                       // restore_rows
                 isl.seeded = isl.accepted_seeded;       // finish_rollback
                 int r, down;
+                isl.trace.push_back({depth, 0, 0, NO_SIGNAL});
                 if (signal_log2(assembled, r)) {
+                    isl.trace.back().signal = r;
                     int q = r - THRESHOLD_LOG2;
                     int fl = (q >= 0) ? q / PER_HALVING_LOG2 : -((-q + PER_HALVING_LOG2 - 1) / PER_HALVING_LOG2);
                     down = fl + 1;
@@ -452,6 +463,7 @@ struct NewtonCtx {
                 depth += down;
                 continue;
             }
+            isl.trace.push_back({depth, converged ? 1 : 0, (converged || finite) ? 1 : 0, NO_SIGNAL});
             if (converged || finite) {
                 if (!converged) isl.stats.unconverged_commits++;
                 commit_span(span, half, converged);
@@ -472,6 +484,7 @@ struct NewtonCtx {
             };
             int r;
             if (remaining > 0 && converged && aligned(depth) && signal_log2(assembled, r)) {
+                isl.trace.back().signal = r;
                 int q = THRESHOLD_LOG2 - r;
                 int up = (q >= 0) ? q / PER_HALVING_LOG2 : -((-q + PER_HALVING_LOG2 - 1) / PER_HALVING_LOG2);
                 while (up > 0 && aligned(depth)) { depth -= 1; up -= 1; }
@@ -490,7 +503,7 @@ inline void newton_step_island(Island& isl, const std::vector<std::array<double,
 
 // Explicit family through the bridge (bridge.rs:19-63): world wrench = external +
 // joints at the current pose and world velocity, pulled back to the body frame.
-enum ExplicitKind { EXPLICIT_EULER = 0, SYMPLECTIC_EULER = 1 };
+enum ExplicitKind { EXPLICIT_EULER = 0, SYMPLECTIC_EULER = 1, LIE_EULER = 3 };
 inline void explicit_step_island(Island& isl, const std::vector<std::array<double, 6>>& ext, double dt, double warp,
                                  int kind) {
     int m = (int)isl.bodies.size();
@@ -508,6 +521,7 @@ inline void explicit_step_island(Island& isl, const std::vector<std::array<doubl
         Body& b = isl.bodies[i];
         M16 w_body = transport_co(screw_from6(total[i].data()), motor_inverse(b.pose));
         if (kind == SYMPLECTIC_EULER) symplectic_euler_step(b, w_body, dt);
+        else if (kind == LIE_EULER) lie_euler_step(b, w_body, dt);
         else explicit_euler_step(b, w_body, dt);
     }
 }
@@ -539,7 +553,7 @@ struct World {
     bool gravity = true;
     double uniform[3] = {0.0, 0.0, 0.0};   // UniformField acceleration (field.rs:16-44), 0 = off
     std::vector<Charge> front;
-    int integrator = 2;                     // 0 ExplicitEuler, 1 SymplecticEuler, 2 Newton
+    int integrator = 2;                     // 0 ExplicitEuler, 1 SymplecticEuler, 2 Newton, 3 LieEuler
     NewtonParams prm;
     int64_t steps = 0;
 

@@ -362,5 +362,25 @@ inline void explicit_euler_step(Body& b, const M16& wrench, double dt) {
     b.pose = compose(twist_exp(v, dt), b.pose);
     b.momentum = b.momentum + (wrench + gyro).scale(dt);
 }
+// Motor::rotation_part (motor.rs:244-249): the scalar and Euclidean-bivector blades,
+// normalized by √⟨M·M̃⟩₀ (motor.rs:231-242; left as is when that is effectively 0).
+inline M16 rotation_part(const M16& m) {
+    M16 r;
+    r.c[0] = m.c[0];
+    for (int b = 1; b < DIM; ++b)
+        if (popc(b) == 2 && !(b & ZERO_MASK)) r.c[b] = m.c[b];
+    double n2 = gp<EVEN, EVEN>(r, reverse(r)).c[0];
+    if (is_effective_zero(n2)) return r;
+    return r.scale(1.0 / std::sqrt(n2));
+}
+// LieEuler (lie_euler.rs:26-56): P += W·dt; δ = exp(dt·V(P)); P ← P transported by
+// rot(δ)⁻¹; M ← δ∘M — the momentum update by conjugation with the pose increment.
+inline void lie_euler_step(Body& b, const M16& wrench, double dt) {
+    b.momentum = b.momentum + wrench.scale(dt);
+    M16 delta = twist_exp(body_velocity(b), dt);
+    M16 drot = rotation_part(delta);
+    b.momentum = transport_co(b.momentum, motor_inverse(drot));
+    b.pose = compose(delta, b.pose);
+}
 
 }  // namespace oracle

@@ -93,6 +93,7 @@ def lib():
         "orc_world_get": (None, [C.c_void_p, _dp, _dp, _dp, _dp]),
         "orc_world_front": (None, [C.c_void_p, _dp]),
         "orc_world_stats": (None, [C.c_void_p, _i64p]),
+        "orc_world_span_trace": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int]),
         "orc_ns_solve": (None, [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]),
         "orc_world_set_uniform": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
         "orc_rawmap_new": (C.c_void_p, [C.c_uint64]),
@@ -289,7 +290,8 @@ def ns_solve(a_blocks, rhs, iters):
 
 
 # ───────────── world ─────────────
-INTEGRATOR = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2}
+INTEGRATOR = {"explicit_euler": 0, "symplectic_euler": 1, "newton": 2, "lie_euler": 3}
+NO_SIGNAL = -100000
 
 
 class OracleWorld:
@@ -341,6 +343,13 @@ class OracleWorld:
         out = np.zeros((self.ni, 6), dtype=np.int64)
         lib().orc_world_stats(self.h, out.ctypes.data_as(_i64p))
         return out
+
+    def span_trace(self, island=0, max_spans=1024):
+        """The spans of `island`'s last step: (depth, converged, committed, signal) rows;
+        signal is NO_SIGNAL when the step read none after that span."""
+        out = np.zeros((max_spans, 4), dtype=np.int32)
+        n = lib().orc_world_span_trace(self.h, int(island), out.ctypes.data_as(C.POINTER(C.c_int32)), max_spans)
+        return out[:min(n, max_spans)].copy()
 
     def close(self):
         if getattr(self, "h", None):
