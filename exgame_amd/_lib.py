@@ -46,6 +46,7 @@ SIGNATURES = {
     "exg_world_set_graph": [C.c_void_p, C.c_int],
     "exg_world_step": [C.c_void_p, C.c_int, C.c_double, C.c_double],
     "exg_world_sync": [C.c_void_p],
+    "exg_world_fatals": [C.c_void_p, C.POINTER(C.c_int64)],
     "exg_world_download": [C.c_void_p, _dp, _dp, _dp],
     "exg_world_charges": [C.c_void_p, _dp],
     "exg_world_external": [C.c_void_p, _dp],

@@ -52,8 +52,9 @@ def inverse_doubles(scene) -> int:
     return int(np.sum(36 * m * m))
 
 
-def footprint(scene, gravity_chunk: int = 1024) -> int:
-    """HBM bytes the scene occupies in an arena."""
+def footprint(scene, gravity_chunk: int = 0) -> int:
+    """HBM bytes the scene occupies in an arena (gravity chunk setting 0: the
+    single-world rule)."""
     out = C.c_int64()
     _lib.check(_bind().exg_arena_footprint(scene.n_bodies, scene.n_islands, scene.n_joints, inverse_doubles(scene),
                                            int(gravity_chunk), C.byref(out)))
@@ -63,7 +64,7 @@ def footprint(scene, gravity_chunk: int = 1024) -> int:
 class Arena:
     def __init__(self, bodies: int, islands: int, joints: int, inverses: int, tol2: float = 1e-20,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
-                 gravity_chunk: int = 1024, jacobian_precision: str = "f64", graph: bool = False):
+                 gravity_chunk: int = 0, jacobian_precision: str = "f64", graph: bool = False):
         self.L = _bind()
         h = C.c_void_p()
         _lib.check(self.L.exg_arena_create(int(bodies), int(islands), int(joints), int(inverses), C.byref(h)))
@@ -77,9 +78,17 @@ class Arena:
         _lib.check(self.L.exg_arena_set_graph(self.h, int(bool(graph))))
         self.shapes = {}
 
+    def set_gravity_chunk(self, chunk: int) -> None:
+        """Sources per gravity chunk of worlds admitted from now on (0: the chunking a
+        stand-alone world of the same size uses); admitted worlds keep theirs."""
+        _lib.check(self.L.exg_arena_set_gravity_chunk(self.h, int(chunk)))
+        self.gravity_chunk = int(chunk)
+
     def autotune_gravity(self) -> int:
-        """Time the gravity pass of the live worlds over chunk lengths 128 … 8192 and
-        keep the fastest; returns it."""
+        """Time the gravity pass of the live worlds under the chunk settings 0 (the
+        single-world rule) and 128 … 8192, and keep the fastest for worlds admitted from
+        now on (admitted worlds keep their chunking, so their results do not change);
+        returns it."""
         out = C.c_int(0)
         _lib.check(self.L.exg_arena_autotune_gravity(self.h, C.byref(out)))
         self.gravity_chunk = out.value

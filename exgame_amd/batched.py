@@ -9,8 +9,9 @@ rollout, the device time per step, and the precision used.
     python -m exgame_amd.batched [--worlds 512] [--steps 20] [--precision f64|f32c]
                                  [--jacobian f64|f32c] [--chunk N | --autotune] [--graph]
 
-`--autotune` times the gravity pass over chunk lengths 128 … 8192 on the admitted
-worlds and keeps the fastest; `--graph` replays two steps at a time from a HIP graph.
+`--autotune` times the gravity pass under the chunk settings (0 = the single-world
+rule, 128 … 8192) on the admitted worlds, then admits the worlds afresh with the
+fastest; `--graph` replays two steps at a time from a HIP graph.
 """
 from __future__ import annotations
 
@@ -54,7 +55,11 @@ def main() -> int:
     if args.autotune:
         ar.step(1, 1e-2, 1.0)     # charges past the bootstrap frame
         ar.sync()
-        ar.autotune_gravity()
+        ar.autotune_gravity()     # applies to admissions from now on: admit the worlds again
+        for i in ids:
+            ar.retire(i)
+        ids = [ar.admit(w) for w in worlds]
+        assert all(i is not None for i in ids)
     ar.step(args.warmup, 1e-2, 1.0)
     ar.sync()
     t0 = time.perf_counter()

@@ -8,11 +8,13 @@
 // HBM arrays handed out first-fit — and one step is one gravity pass over a work
 // table of all worlds (each world's sources stay its own) plus one launch per
 // island class over the island ids of all worlds. Nothing is exchanged between
-// worlds, and every per-world quantity (gravity chunking, island kernel, solver
-// budgets) depends only on that world, so a world's trajectory is bit-identical
+// worlds, and every per-world quantity (island kernel, solver budgets, and the
+// gravity chunking, fixed when the world is admitted) depends only on that world and
+// the arena's settings at its admission, so a world's trajectory is bit-identical
 // alone or in any mix.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 #include "../../include/exgame.h"
@@ -83,6 +85,7 @@ struct ArenaWorld {
     long long b0 = 0, nb = 0, i0 = 0, ni = 0, j0 = 0, nj = 0, x0 = 0, nx = 0;
     double G = 1.0;
     int gravity_on = 1;
+    int chunk = 0;   // sources per gravity chunk, fixed at admission (it sets the order of the force sums)
 };
 
 }  // namespace
@@ -91,12 +94,14 @@ struct exg_arena {
     int Bc = 0, Ic = 0, Jc = 0;
     long long Xc = 0;
     int cus = 256;
-    int grav_chunk = 1024;
+    int grav_chunk = 0;   // chunk length of worlds admitted from now on; 0: the single-world rule
     double tol2 = 1e-8;
     int ns_warm = 2, ns_seed = 24, ns_engine = EXG_NS_MFMA, grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     hipStream_t stream = nullptr;
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
+    DevBuf<int> fatal;                // island-steps that ended with a non-finite state
+    long long fatal_reported = 0;
     DevBuf<long long> xacc_off;
     DevBuf<exg::IslandTopo> topo;
     DevBuf<exg::GravEntry> entries;
@@ -122,7 +127,8 @@ struct exg_arena {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
                         &partial, &island_G})
             b->release();
-        for (auto* b : {&island_start, &island_end, &island_jstart, &island_jend, &jkind, &ja, &jb, &seeded, &stats})
+        for (auto* b : {&island_start, &island_end, &island_jstart, &island_jend, &jkind, &ja, &jb, &seeded, &stats,
+                        &fatal})
             b->release();
         xacc_off.release();
         topo.release();
@@ -137,15 +143,27 @@ struct exg_arena {
         stream = nullptr;
     }
 
-    // Gravity work table and island-class id lists of the live worlds.
-    int rebuild() {
+    // Sources per chunk for a world of n bodies under chunk setting `setting`: a fixed
+    // length, or (0) what a stand-alone world of n bodies uses (exg_world_create).
+    int chunk_for(long long n, int setting) const {
+        if (setting > 0) return setting;
+        std::vector<exg::GravEntry> en;
+        std::vector<exg::GravTile> tl;
+        exg::gravity_add_world(en, tl, 0, (int)n, 1.0, cus);
+        return en.empty() ? (int)n : en[0].src1 - en[0].src0;
+    }
+    // Gravity work table and island-class id lists of the live worlds; with
+    // `override_chunk` ≥ 0 every world is chunked by that setting instead of its own
+    // (timing only: exg_arena_autotune_gravity).
+    int rebuild(int override_chunk = -1) {
         std::vector<exg::GravEntry> en;
         std::vector<exg::GravTile> tl;
         std::vector<int> ids[exg::ISLAND_CLASSES];
         for (auto& w : worlds) {
             if (!w.live) continue;
             if (w.gravity_on) {
-                exg::gravity_add_world_chunked(en, tl, (int)w.b0, (int)w.nb, w.G, grav_chunk);
+                const int ch = override_chunk >= 0 ? chunk_for(w.nb, override_chunk) : w.chunk;
+                exg::gravity_add_world_chunked(en, tl, (int)w.b0, (int)w.nb, w.G, ch);
             } else {
                 // tiles without entries: the reduce writes a zero wrench
                 for (long long t = 0; t < w.nb; t += exg::GRAV_TILE) {
@@ -235,6 +253,8 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
     R_TRY(r->island_jend.alloc(I));
     R_TRY(r->seeded.alloc(I));
     R_TRY(r->stats.alloc(I * exg::ISLAND_STATS));
+    R_TRY(r->fatal.alloc(1));
+    R_TRY(hipMemset(r->fatal.p, 0, sizeof(int)));
     R_TRY(r->xacc_off.alloc(I));
     R_TRY(r->topo.alloc(I));
     R_TRY(r->xacc.alloc((size_t)inverse_capacity));
@@ -287,9 +307,9 @@ int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian) {
 
 int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_gravity_chunk: null arena");
-    if (chunk_len < 64) return afail(EXG_ERR_INVALID, "exg_arena_set_gravity_chunk: chunk below 64 sources");
-    r->grav_chunk = chunk_len;
-    r->dirty = true;
+    if (chunk_len != 0 && chunk_len < 64)
+        return afail(EXG_ERR_INVALID, "exg_arena_set_gravity_chunk: chunk below 64 sources");
+    r->grav_chunk = chunk_len;   // worlds already admitted keep their chunking
     return EXG_OK;
 }
 
@@ -308,6 +328,9 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     int max_m = 0, max_j = 0;
     if (int rc = exg::check_world("exg_arena_admit", n_bodies, n_islands, island_start, n_joints, kind, a, b, jstart,
                                   max_m, max_j))
+        return rc;
+    if (int rc = exg::check_bodies("exg_arena_admit", n_bodies, pose, momentum, mass, inertia, size, n_islands, origin,
+                                   n_joints, kind, params, G))
         return rc;
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
@@ -335,8 +358,22 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     w.i0 = r->islands.take(n_islands);
     w.j0 = r->joints.take(n_joints);
     w.x0 = r->xaccs.take(nx);
+    // every failure from here on gives the four ranges back
+    struct Release {
+        exg_arena* r;
+        const ArenaWorld& w;
+        bool armed = true;
+        ~Release() {
+            if (!armed) return;
+            r->bodies.give(w.b0, w.nb);
+            r->islands.give(w.i0, w.ni);
+            r->joints.give(w.j0, w.nj);
+            r->xaccs.give(w.x0, w.nx);
+        }
+    } release_on_failure{r, w};
     w.G = G;
     w.gravity_on = gravity_on != 0;
+    w.chunk = r->chunk_for(n_bodies, r->grav_chunk);
     const size_t B = (size_t)n_bodies, I = (size_t)n_islands, J = (size_t)n_joints;
     auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
     A_TRY(up(r->pose.p + w.b0 * 8, pose, B * 8 * sizeof(double)));
@@ -387,6 +424,7 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     // ids are never reused, so a retired world's id stays invalid
     const int id = (int)r->worlds.size();
     r->worlds.push_back(w);
+    release_on_failure.armed = false;
     r->dirty = true;
     *world_id = id;
     return EXG_OK;
@@ -439,6 +477,7 @@ static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
     a.seeded = r->seeded.p;
     a.charges_out = back;
     a.stats = r->stats.p;
+    a.fatal = r->fatal.p;
     a.dt = dt;
     a.warp = warp;
     a.tol2 = r->tol2;
@@ -494,6 +533,8 @@ static int arena_graph(exg_arena* r, double dt, double warp, hipGraphExec_t* out
 int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_step: null arena");
     if (n_steps < 0) return afail(EXG_ERR_INVALID, "exg_arena_step: negative step count");
+    if (!(dt > 0.0) || !std::isfinite(dt) || !(warp >= 1.0) || !std::isfinite(warp))
+        return afail(EXG_ERR_INVALID, "exg_arena_step: dt must be positive and warp at least 1 (both finite)");
     if (r->dirty)
         if (int rc = r->rebuild()) return rc;
     bool any = false;
@@ -520,14 +561,14 @@ int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
         if (chosen_chunk) *chosen_chunk = r->grav_chunk;
         return EXG_OK;
     }
-    // each candidate: its work table, one warm pass, then the mean of five timed passes
-    // of the gravity kernels on the current charges (the next step recomputes ext)
-    static const int candidates[] = {128, 256, 512, 1024, 2048, 4096, 8192};
+    // each candidate setting: the live worlds' work table chunked by it, one warm pass,
+    // then the mean of five timed passes on the current charges (ext is recomputed by
+    // the next step); the worlds' own chunking is restored afterwards
+    static const int candidates[] = {0, 128, 256, 512, 1024, 2048, 4096, 8192};
     int best = r->grav_chunk;
     float best_ms = -1.0f;
     for (int c : candidates) {
-        r->grav_chunk = c;
-        if (int rc = r->rebuild()) return rc;
+        if (int rc = r->rebuild(c)) return rc;
         double* front = r->charges[r->front].p;
         exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
                             r->partial.p, r->ext.p, r->stream);
@@ -545,7 +586,7 @@ int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
             best = c;
         }
     }
-    r->grav_chunk = best;
+    r->grav_chunk = best;   // for worlds admitted from now on
     if (int rc = r->rebuild()) return rc;
     if (chosen_chunk) *chosen_chunk = best;
     return EXG_OK;
@@ -560,6 +601,15 @@ int exg_arena_set_graph(exg_arena* r, int enable) {
 int exg_arena_sync(exg_arena* r) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_sync: null arena");
     A_TRY(hipStreamSynchronize(r->stream));
+    int c = 0;
+    A_TRY(hipMemcpy(&c, r->fatal.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (c > r->fatal_reported) {
+        const long long fresh = c - r->fatal_reported;
+        r->fatal_reported = c;
+        return afail(EXG_ERR_FATAL, "exg_arena_sync: " + std::to_string(fresh) +
+                                        " island-step(s) ended with a non-finite state (per-island counts in "
+                                        "exg_arena_world_stats slot 6)");
+    }
     return EXG_OK;
 }
 
@@ -605,8 +655,17 @@ int exg_arena_usage(exg_arena* r, int64_t* used4, int64_t* capacity4) {
 
 int exg_arena_footprint(int n_bodies, int n_islands, int n_joints, long long inverse_doubles, int gravity_chunk,
                         int64_t* bytes) {
-    if (!bytes || n_bodies < 0 || n_islands < 0 || n_joints < 0 || inverse_doubles < 0 || gravity_chunk < 64)
+    if (!bytes || n_bodies < 0 || n_islands < 0 || n_joints < 0 || inverse_doubles < 0 ||
+        (gravity_chunk != 0 && gravity_chunk < 64))
         return afail(EXG_ERR_INVALID, "exg_arena_footprint: bad arguments");
+    if (gravity_chunk == 0) {   // the single-world rule (at most 64 workgroups per CU of 256)
+        std::vector<exg::GravEntry> en;
+        std::vector<exg::GravTile> tl;
+        int cus = 256;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) cus = 256;
+        exg::gravity_add_world(en, tl, 0, std::max(n_bodies, 1), 1.0, cus);
+        gravity_chunk = en.empty() ? std::max(n_bodies, 1) : std::max(1, en[0].src1 - en[0].src0);
+    }
     const long long per_body = (8 + 6 + 1 + 9 + 1 + 6 + 16) * 8;                 // state, ext, two charges
     const long long per_island = 3 * 8 + 5 * 4 + exg::ISLAND_STATS * 4 + 8 +      // origin, ranges, stats, offset,
                                  (long long)sizeof(exg::IslandTopo);             // topology record

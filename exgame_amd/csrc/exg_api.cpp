@@ -2,6 +2,7 @@
 // per-step launch sequence, error mapping. Compiled by hipcc into libexgame_hip.so.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -73,6 +74,48 @@ int exg::check_world(const char* who, int n_bodies, int n_islands, const int* is
     return EXG_OK;
 }
 
+int exg::check_bodies(const char* who, int n_bodies, const double* pose, const double* momentum, const double* mass,
+                      const double* inertia, const double* size, int n_islands, const double* origin, int n_joints,
+                      const int* kind, const double* params, double G) {
+    auto bad = [&](const std::string& what, long long i) {
+        return fail(EXG_ERR_INVALID, std::string(who) + ": " + what + " (index " + std::to_string(i) + ")");
+    };
+    if (!std::isfinite(G)) return fail(EXG_ERR_INVALID, std::string(who) + ": G is not finite");
+    for (long long i = 0; i < n_bodies; ++i) {
+        // Inertia::Kinematic (crates/newton/src/inertia.rs:12-45) has no mass: its body is
+        // a prescribed reference, which this library's solver does not model
+        if (mass[i] == 0.0)
+            return fail(EXG_ERR_UNSUPPORTED, std::string(who) + ": kinematic (zero-mass) body " + std::to_string(i) +
+                                                 " is not supported");
+        if (!(mass[i] > 0.0) || !std::isfinite(mass[i])) return bad("mass must be positive and finite", i);
+        const double* t = inertia + 9 * i;
+        for (int k = 0; k < 9; ++k)
+            if (!std::isfinite(t[k])) return bad("inertia tensor is not finite", i);
+        // positive definite (Sylvester: leading minors > 0): the tensor is inverted per step
+        const double m1 = t[0], m2 = t[0] * t[4] - t[1] * t[3];
+        const double m3 = t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6]) +
+                          t[2] * (t[3] * t[7] - t[4] * t[6]);
+        if (!(m1 > 0.0 && m2 > 0.0 && m3 > 0.0)) return bad("inertia tensor is not positive definite", i);
+        const double* p = pose + 8 * i;
+        double r2 = 0.0;
+        for (int k = 0; k < 8; ++k)
+            if (!std::isfinite(p[k])) return bad("pose is not finite", i);
+        for (int k = 0; k < 4; ++k) r2 += p[k] * p[k];   // 1, e12, e13, e23: the rotor part
+        if (!(r2 > 0.0)) return bad("pose has no rotor part", i);
+        for (int k = 0; k < 6; ++k)
+            if (!std::isfinite(momentum[6 * i + k])) return bad("momentum is not finite", i);
+        if (!std::isfinite(size[i]) || size[i] < 0.0) return bad("size must be finite and non-negative", i);
+    }
+    for (long long i = 0; i < 3LL * n_islands; ++i)
+        if (!std::isfinite(origin[i])) return bad("island origin is not finite", i / 3);
+    for (long long q = 0; q < n_joints; ++q) {
+        const int np = (kind[q] == 0 || kind[q] == 1) ? 10 : (kind[q] == 2 ? 2 : 1);
+        for (int k = 0; k < np; ++k)
+            if (!std::isfinite(params[10 * q + k])) return bad("joint parameter is not finite", q);
+    }
+    return EXG_OK;
+}
+
 
 struct exg_world {
     int B = 0, I = 0, J = 0;
@@ -91,6 +134,8 @@ struct exg_world {
     DevBuf<exg::GravTile> grav_tiles;
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
+    DevBuf<int> fatal;              // island-steps that ended with a non-finite state
+    long long fatal_reported = 0;   // the count the last exg_world_sync reported
     DevBuf<long long> xacc_off;
     DevBuf<exg::IslandTopo> topo;   // per island, Newton integrator only
     int front = 0;
@@ -110,7 +155,7 @@ struct exg_world {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
                         &grav_partial})
             b->release();
-        for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats}) b->release();
+        for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats, &fatal}) b->release();
         grav_entries.release();
         grav_tiles.release();
         xacc_off.release();
@@ -157,6 +202,9 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     int max_m = 0, max_j = 0;
     if (int rc = exg::check_world("exg_world_create", n_bodies, n_islands, island_start, n_joints, kind, a, b, jstart,
                                   max_m, max_j))
+        return rc;
+    if (int rc = exg::check_bodies("exg_world_create", n_bodies, pose, momentum, mass, inertia, size, n_islands, origin,
+                                   n_joints, kind, params, G))
         return rc;
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
@@ -210,6 +258,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(w->island_jstart.alloc(I + 1));
     W_TRY(w->seeded.alloc(I));
     W_TRY(w->stats.alloc(I * exg::ISLAND_STATS));
+    W_TRY(w->fatal.alloc(1));
     W_TRY(w->xacc_off.alloc(I));
     {
         std::vector<int> ids[exg::ISLAND_CLASSES];
@@ -264,6 +313,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(hipMemset(w->xacc.p, 0, (size_t)tot * sizeof(double)));
     W_TRY(hipMemset(w->seeded.p, 0, I * sizeof(int)));
     W_TRY(hipMemset(w->stats.p, 0, I * exg::ISLAND_STATS * sizeof(int)));
+    W_TRY(hipMemset(w->fatal.p, 0, sizeof(int)));
     W_TRY(hipMemset(w->ext.p, 0, B * 6 * sizeof(double)));
 #undef W_TRY
     *out = w;
@@ -351,6 +401,7 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.seeded = w->seeded.p;
     a.charges_out = back;
     a.stats = w->stats.p;
+    a.fatal = w->fatal.p;
     a.dt = dt;
     a.warp = warp;
     a.tol2 = w->tol2;
@@ -423,6 +474,9 @@ static int world_graph(exg_world* w, double dt, double warp, hipGraphExec_t* out
 int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_step: null world");
     if (n_steps < 0) return fail(EXG_ERR_INVALID, "exg_world_step: negative step count");
+    // the engine's input facts (crates/newton/build.rs input_facts): dt > 0, warp ≥ 1
+    if (!(dt > 0.0) || !std::isfinite(dt) || !(warp >= 1.0) || !std::isfinite(warp))
+        return fail(EXG_ERR_INVALID, "exg_world_step: dt must be positive and warp at least 1 (both finite)");
     w->ev_used = 0;
     w->graph_ms_valid = false;
     if (w->use_graph && n_steps >= 2) {
@@ -466,9 +520,34 @@ int exg_world_set_graph(exg_world* w, int enable) {
     return EXG_OK;
 }
 
+// Island-steps since creation whose state ended non-finite (after a synchronization).
+static int world_fatal_count(exg_world* w, long long& count) {
+    int c = 0;
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    HIP_TRY(hipMemcpy(&c, w->fatal.p, sizeof(int), hipMemcpyDeviceToHost));
+    count = c;
+    return EXG_OK;
+}
+
 int exg_world_sync(exg_world* w) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_sync: null world");
-    HIP_TRY(hipStreamSynchronize(w->stream));
+    long long c = 0;
+    if (int rc = world_fatal_count(w, c)) return rc;
+    if (c > w->fatal_reported) {
+        const long long fresh = c - w->fatal_reported;
+        w->fatal_reported = c;
+        return fail(EXG_ERR_FATAL, "exg_world_sync: " + std::to_string(fresh) +
+                                       " island-step(s) ended with a non-finite state (per-island counts in "
+                                       "exg_world_stats slot 6)");
+    }
+    return EXG_OK;
+}
+
+int exg_world_fatals(exg_world* w, int64_t* count) {
+    if (!w || !count) return fail(EXG_ERR_INVALID, "exg_world_fatals: null argument");
+    long long c = 0;
+    if (int rc = world_fatal_count(w, c)) return rc;
+    *count = c;
     return EXG_OK;
 }
 

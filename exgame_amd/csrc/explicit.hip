@@ -134,6 +134,12 @@ __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs 
         for (int k = 0; k < 8; ++k) a.pose[(size_t)(b0 + i) * 8 + k] = pose[i][k];
         for (int k = 0; k < 6; ++k) a.mom[(size_t)(b0 + i) * 6 + k] = mom[i][k];
     }
+    bool bad = false;
+    for (int i = 0; i < m; ++i) {
+        for (int k = 0; k < 8; ++k) bad |= !isfinite(pose[i][k]);
+        for (int k = 0; k < 6; ++k) bad |= !isfinite(mom[i][k]);
+    }
+    if (bad) mark_fatal(a, isl);
 }
 
 void launch_explicit_step(const IslandArgs& a, int integrator, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {

@@ -352,6 +352,21 @@ def main():
         body.append(f"ibv[{k}] = {sub_index(poly_c(ib[blade]), 't', 6)};")
     em.fn("void exp_parts(const S t[6], S& u, S& p, S bv[8], S ibv[8])", body)
 
+    # the reference's Pod layout of a screw: its grade-2 store, blades ascending
+    # {e12, e13, e23, e01, e02, e03} with the markup signs (what
+    # crates/newton/build.rs:106-152 probes at build time), to and from (lin, ang)
+    cs = sp.symbols("c0:6")
+    Sm = screw(cs[:3], cs[3:])
+    em.fn("void screw_to_pod(const S c[6], S pod[6])",
+          [f"pod[{k}] = {sub_index(poly_c(Sm[blade]), 'c', 6)};" for k, blade in enumerate(G2)])
+    qs = sp.symbols("q0:6")
+    Qm = [sp.Integer(0)] * 16
+    for k, blade in enumerate(G2):
+        Qm[blade] = qs[k]
+    back = screw_read(Qm)
+    em.fn("void screw_from_pod(const S q[6], S c[6])",
+          [f"c[{i}] = {sub_index(poly_c(back[i]), 'q', 6)};" for i in range(6)])
+
     hdr = [
         "// GENERATED by gen_pga3.py — do not edit by hand.",
         "// Closed-form PGA3 operations (exact polynomial expansions of the reference's",

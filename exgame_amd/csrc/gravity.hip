@@ -271,9 +271,8 @@ __global__ __launch_bounds__(ZERO_T) void zero_wrench_kernel(double* __restrict_
     if (t < n * 6) ext[(size_t)b0 * 6 + t] = 0.0;
 }
 
-void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
-                       int compute_units) {
-    if (n <= 0) return;
+int gravity_chunk_len(int n, int compute_units) {
+    if (n <= 0) return 1;
     // enough workgroups to fill the chip (4 per CU), chunks of at most 1,024 sources
     // (long chunks leave the later waves idle), at most 64 workgroups per CU (the
     // partial sums stay small), at least 64 sources per chunk
@@ -285,7 +284,13 @@ void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& t
     const int max_chunks = (n + GRAV_MIN_CHUNK - 1) / GRAV_MIN_CHUNK;
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
-    gravity_add_world_chunked(entries, tiles, b0, n, G, (n + chunks - 1) / chunks);
+    return (n + chunks - 1) / chunks;
+}
+
+void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
+                       int compute_units) {
+    if (n <= 0) return;
+    gravity_add_world_chunked(entries, tiles, b0, n, G, gravity_chunk_len(n, compute_units));
 }
 
 void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,

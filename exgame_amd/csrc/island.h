@@ -16,6 +16,14 @@ int set_error(int code, const char* msg);
 int check_world(const char* who, int n_bodies, int n_islands, const int* island_start, int n_joints,
                 const int* kind, const int* a, const int* b, std::vector<int>& jstart, int& max_m, int& max_j);
 
+// Validates the state of a world's bodies, islands and joints: positive finite masses
+// (zero mass, a kinematic body, gives EXG_ERR_UNSUPPORTED), positive definite finite
+// inertia tensors, finite poses with a rotor part, finite momenta, sizes, origins,
+// joint parameters and G. Returns EXG_OK or the error (message recorded).
+int check_bodies(const char* who, int n_bodies, const double* pose, const double* momentum, const double* mass,
+                 const double* inertia, const double* size, int n_islands, const double* origin, int n_joints,
+                 const int* kind, const double* params, double G);
+
 // Tables of one island that depend only on its joint graph (neighbour lists, assembly
 // terms, incidence, joints by kind, A's non-zero slab masks and the MFMA tile lists
 // of each wave), built on the host by build_island_topology (topology.cpp) when a
@@ -38,7 +46,11 @@ int build_island_topology(int m, int J, const int* ja, const int* jb, const int*
                           IslandTopo& t);
 
 constexpr int ISLAND_THREADS = 256;
-constexpr int ISLAND_STATS = 8;  // spans, newton iterations, NS iterations, reseeds, line-search rejects, unconverged commits
+// Per-island counters: spans, Newton iterations, NS iterations, reseeds, line-search
+// rejects, unconverged commits, steps that ended with a non-finite state (the fatal
+// mark), spans found non-finite at the depth limit and not committed.
+constexpr int ISLAND_STATS = 8;
+constexpr int STAT_FATAL = 6, STAT_NONFINITE_SPANS = 7;
 
 // One implicit step of every island (one workgroup per island).
 struct IslandArgs {
@@ -69,7 +81,15 @@ struct IslandArgs {
     const double* G_of_island;// per-island gravitational constant for the published charges; null: G
     int jac_f32;              // 1: FP32 Jacobian lanes and compensated FP32 assembly (mixed precision)
     const IslandTopo* topo;   // I records (islands of the single-body class need none)
+    int* fatal;               // island-steps that ended with a non-finite state (one counter); may be null
 };
+
+// Mark-and-continue fatal report of one island-step (the state is written as it is):
+// the island's STAT_FATAL counter and the launch-wide counter.
+__device__ inline void mark_fatal(const IslandArgs& a, int isl) {
+    a.stats[(size_t)isl * ISLAND_STATS + STAT_FATAL] += 1;
+    if (a.fatal) atomicAdd(a.fatal, 1);
+}
 
 // Island classes of the fused Newton kernel and the launch of one class over the
 // island ids in a.island_ids[0 .. a.n_islands).
@@ -115,6 +135,8 @@ constexpr int GRAV_TILE = 64;                     // targets per tile
 constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * GRAV_TILE;   // doubles of partial sums per entry
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units);
+// The chunk length gravity_add_world uses for a world of n bodies (a pure function).
+int gravity_chunk_len(int n, int compute_units);
 // The same with a fixed chunk length (≥ 64 sources).
 void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                                int chunk_len);

@@ -135,7 +135,7 @@ struct IslandSolver {
     bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
     double dt, warp, half, span;
     long long xoff;
-    int st_spans, st_newton, st_ns, st_reseed, st_reject, st_uncommit;
+    int st_spans, st_newton, st_ns, st_reseed, st_reject, st_uncommit, st_nonfinite;
     bool seeded;
 
     // the thread index as an opaque value at each use, so that index arithmetic is
@@ -849,6 +849,7 @@ struct IslandSolver {
             } else {
                 restore_x();
                 seeded = accepted_seeded;
+                st_nonfinite++;
             }
             remaining -= units;
             auto aligned = [&](int d) {
@@ -944,7 +945,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     sv.dt = a.dt;
     sv.warp = a.warp;
     sv.xoff = xoff;
-    sv.st_spans = sv.st_newton = sv.st_ns = sv.st_reseed = sv.st_reject = sv.st_uncommit = 0;
+    sv.st_spans = sv.st_newton = sv.st_ns = sv.st_reseed = sv.st_reject = sv.st_uncommit = sv.st_nonfinite = 0;
     sv.seeded = a.seeded[isl] != 0;
     sv.solve_step(a.tol2, a.ns_warm, a.ns_seed);
     __syncthreads();
@@ -972,6 +973,12 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         }
         for (int k = 0; k < 3; ++k) s.bcast[2 + k] = a.origin[isl * 3 + k];
         s.ibcast[1] = sv.seeded ? 1 : 0;
+        bool bad = false;
+        for (int i = 0; i < m; ++i) {
+            for (int k = 0; k < 8; ++k) bad |= !isfinite(s.pose[i][k]);
+            for (int k = 0; k < 6; ++k) bad |= !isfinite(s.mom[i][k]);
+        }
+        s.ibcast[2] = bad ? 1 : 0;
     }
     __syncthreads();
     double org[3] = {s.bcast[2], s.bcast[3], s.bcast[4]};
@@ -998,6 +1005,8 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         st[3] += sv.st_reseed;
         st[4] += sv.st_reject;
         st[5] += sv.st_uncommit;
+        st[STAT_NONFINITE_SPANS] += sv.st_nonfinite;
+        if (s.ibcast[2]) mark_fatal(a, isl);
     }
 }
 

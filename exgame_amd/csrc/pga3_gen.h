@@ -197,4 +197,24 @@ __host__ __device__ inline void exp_parts(const S t[6], S& u, S& p, S bv[8], S i
     ibv[7] = S(0.0);
 }
 
+template <class S>
+__host__ __device__ inline void screw_to_pod(const S c[6], S pod[6]) {
+    pod[0] = -c[5];
+    pod[1] = c[4];
+    pod[2] = -c[3];
+    pod[3] = -c[0];
+    pod[4] = -c[1];
+    pod[5] = -c[2];
+}
+
+template <class S>
+__host__ __device__ inline void screw_from_pod(const S q[6], S c[6]) {
+    c[0] = -q[3];
+    c[1] = -q[4];
+    c[2] = -q[5];
+    c[3] = -q[2];
+    c[4] = q[1];
+    c[5] = -q[0];
+}
+
 }  // namespace exg

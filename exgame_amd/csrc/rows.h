@@ -29,17 +29,21 @@ enum Kind : int {
     N_KINDS = 20
 };
 
+// Device storages of one store; every non-row storage holds the same scalar type
+// (scalar_bytes 4: f32, 8: f64). `fatal` counts rows with non-finite outputs.
 struct StorePtrs {
-    double* scalar;
-    double* motor;
-    double* twist;
-    double* wrench;
-    double* pair;
-    double* jac;
-    double* block;
-    double* diag;
-    double* full;
+    void* scalar;
+    void* motor;
+    void* twist;
+    void* wrench;
+    void* pair;
+    void* jac;
+    void* block;
+    void* diag;
+    void* full;
     const uint32_t* row;
+    unsigned* fatal;
+    int scalar_bytes;
 };
 
 void launch_rows(int kind, const StorePtrs& s, const uint32_t* row_slots, uint32_t count, hipStream_t stream);

@@ -60,7 +60,7 @@ struct Single {
     double Ar[6], Xr[6], Tr[6], Mr[6];  // row `row` of A, X, T and the candidate-0 mass block
     double dt, warp, half, span;
     bool seeded;
-    int st_spans = 0, st_newton = 0, st_ns = 0, st_reseed = 0, st_reject = 0, st_uncommit = 0;
+    int st_spans = 0, st_newton = 0, st_ns = 0, st_reseed = 0, st_reject = 0, st_uncommit = 0, st_nonfinite = 0;
 
     // Midpoint, residual and residual ratio of a candidate velocity (pre + gather +
     // body_post + wnorm for one body without joints; functions/src/lib.rs:170-247,
@@ -310,6 +310,7 @@ struct Single {
             } else {
                 load_x(xoff);
                 seeded = accepted_seeded;
+                st_nonfinite++;
             }
             remaining -= units;
             auto aligned = [&](int d) {
@@ -391,6 +392,11 @@ __global__ __launch_bounds__(SB) void singles_step_kernel(IslandArgs a) {
     st[3] += sg.st_reseed;
     st[4] += sg.st_reject;
     st[5] += sg.st_uncommit;
+    st[STAT_NONFINITE_SPANS] += sg.st_nonfinite;
+    bool bad = false;
+    for (int q = 0; q < 8; ++q) bad |= !isfinite(sg.pose[q]);
+    for (int q = 0; q < 6; ++q) bad |= !isfinite(sg.mom[q]);
+    if (bad) mark_fatal(a, isl);
 }
 
 }  // namespace

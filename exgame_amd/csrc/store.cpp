@@ -25,9 +25,12 @@ namespace {
 
 int sfail(int code, const std::string& msg) { return exg::set_error(code, msg.c_str()); }
 
-constexpr size_t ELEM_BYTES[exg::N_STORE_TYPES] = {
-    8, 8 * 8, 6 * 8, 6 * 8, 12 * 8, 288 * 8, 36 * 8, 3 * 8, 9 * 8, exg::ROW_WORDS * 4,
-};
+// scalars per element of each storage (the ROW storage holds 128 u32 words)
+constexpr size_t ELEM_SCALARS[exg::N_STORE_TYPES] = {1, 8, 6, 6, 12, 288, 36, 3, 9, 0};
+
+size_t elem_bytes(int type, int scalar_bytes) {
+    return type == exg::T_ROW ? exg::ROW_WORDS * 4 : ELEM_SCALARS[type] * (size_t)scalar_bytes;
+}
 
 struct SlotKey {
     uint32_t is_value, next, generation;
@@ -47,9 +50,12 @@ struct TypedMap {
 
 struct exg_store {
     TypedMap maps[exg::N_STORE_TYPES];
+    int scalar_bytes = 8;                 // 4: f32 storages, 8: f64
     hipStream_t stream = nullptr;
     uint32_t* dev_slots = nullptr;
     size_t dev_slots_cap = 0;
+    unsigned* dev_fatal = nullptr;        // rows of the running dispatch with non-finite outputs
+    uint32_t fatals[exg::N_KINDS] = {};   // that count for the last dispatch of each kind
 };
 
 namespace {
@@ -180,14 +186,19 @@ bool row_refs(int kind, const uint32_t* w, std::vector<SlotRef>& out) {
 
 extern "C" {
 
-int exg_store_create(uint64_t capacity, exg_store** out) {
+int exg_store_create_scalar(uint64_t capacity, int scalar, exg_store** out) {
     if (!out) return sfail(EXG_ERR_INVALID, "exg_store_create: null output");
+    *out = nullptr;
     if (capacity < 1) return sfail(EXG_ERR_INVALID, "exg_store_create: world storage capacity must be at least 1");
+    if (scalar != EXG_SCALAR_F32 && scalar != EXG_SCALAR_F64)
+        return sfail(EXG_ERR_INVALID, "exg_store_create: storage scalar must be EXG_SCALAR_F32 or EXG_SCALAR_F64");
     auto* s = new exg_store();
+    s->scalar_bytes = scalar == EXG_SCALAR_F32 ? 4 : 8;
     hipError_t e = hipStreamCreate(&s->stream);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&s->dev_fatal), sizeof(unsigned));
     for (int t = 0; t < exg::N_STORE_TYPES && e == hipSuccess; ++t) {
         TypedMap& m = s->maps[t];
-        m.elem = ELEM_BYTES[t];
+        m.elem = elem_bytes(t, s->scalar_bytes);
         m.capacity = capacity;
         e = hipMalloc(&m.dev, capacity * m.elem);
         if (e == hipSuccess) e = hipMemset(m.dev, 0, capacity * m.elem);
@@ -195,6 +206,7 @@ int exg_store_create(uint64_t capacity, exg_store** out) {
     }
     if (e != hipSuccess) {
         for (auto& m : s->maps) if (m.dev) (void)hipFree(m.dev);
+        if (s->dev_fatal) (void)hipFree(s->dev_fatal);
         if (s->stream) (void)hipStreamDestroy(s->stream);
         delete s;
         return sfail(EXG_ERR_BACKEND, std::string("exg_store_create: ") + hipGetErrorString(e));
@@ -203,11 +215,20 @@ int exg_store_create(uint64_t capacity, exg_store** out) {
     return EXG_OK;
 }
 
+int exg_store_create(uint64_t capacity, exg_store** out) { return exg_store_create_scalar(capacity, EXG_SCALAR_F64, out); }
+
+int exg_store_scalar(exg_store* s, int* scalar) {
+    if (!s || !scalar) return sfail(EXG_ERR_INVALID, "exg_store_scalar: null argument");
+    *scalar = s->scalar_bytes == 4 ? EXG_SCALAR_F32 : EXG_SCALAR_F64;
+    return EXG_OK;
+}
+
 int exg_store_destroy(exg_store* s) {
     if (!s) return EXG_OK;
     (void)hipStreamSynchronize(s->stream);
     for (auto& m : s->maps) if (m.dev) (void)hipFree(m.dev);
     if (s->dev_slots) (void)hipFree(s->dev_slots);
+    if (s->dev_fatal) (void)hipFree(s->dev_fatal);
     (void)hipStreamDestroy(s->stream);
     delete s;
     return EXG_OK;
@@ -348,24 +369,33 @@ int exg_dispatch(exg_store* s, int kind, const uint32_t* row_slots, uint32_t cou
         s->dev_slots_cap = cap;
     }
     hipError_t e = hipMemcpyAsync(s->dev_slots, row_slots, count * sizeof(uint32_t), hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->dev_fatal, 0, sizeof(unsigned), s->stream);
     if (e != hipSuccess) return sfail(EXG_ERR_BACKEND, std::string("exg_dispatch: ") + hipGetErrorString(e));
     exg::StorePtrs p;
-    p.scalar = static_cast<double*>(s->maps[exg::T_SCALAR].dev);
-    p.motor = static_cast<double*>(s->maps[exg::T_MOTOR].dev);
-    p.twist = static_cast<double*>(s->maps[exg::T_TWIST].dev);
-    p.wrench = static_cast<double*>(s->maps[exg::T_WRENCH].dev);
-    p.pair = static_cast<double*>(s->maps[exg::T_WRENCH_PAIR].dev);
-    p.jac = static_cast<double*>(s->maps[exg::T_JACOBIAN].dev);
-    p.block = static_cast<double*>(s->maps[exg::T_BLOCK].dev);
-    p.diag = static_cast<double*>(s->maps[exg::T_DIAG_INERTIA].dev);
-    p.full = static_cast<double*>(s->maps[exg::T_FULL_INERTIA].dev);
+    p.scalar = s->maps[exg::T_SCALAR].dev;
+    p.motor = s->maps[exg::T_MOTOR].dev;
+    p.twist = s->maps[exg::T_TWIST].dev;
+    p.wrench = s->maps[exg::T_WRENCH].dev;
+    p.pair = s->maps[exg::T_WRENCH_PAIR].dev;
+    p.jac = s->maps[exg::T_JACOBIAN].dev;
+    p.block = s->maps[exg::T_BLOCK].dev;
+    p.diag = s->maps[exg::T_DIAG_INERTIA].dev;
+    p.full = s->maps[exg::T_FULL_INERTIA].dev;
     p.row = static_cast<const uint32_t*>(s->maps[exg::T_ROW].dev);
+    p.fatal = s->dev_fatal;
+    p.scalar_bytes = s->scalar_bytes;
     exg::launch_rows(kind, p, s->dev_slots, count, s->stream);
     e = hipGetLastError();
     if (e != hipSuccess) return sfail(EXG_ERR_BACKEND, std::string("exg_dispatch: ") + hipGetErrorString(e));
     // the slot list is staged in one buffer: wait so the next dispatch can reuse it
     e = hipStreamSynchronize(s->stream);
+    unsigned fatal = 0;
+    if (e == hipSuccess) e = hipMemcpy(&fatal, s->dev_fatal, sizeof(unsigned), hipMemcpyDeviceToHost);
     if (e != hipSuccess) return sfail(EXG_ERR_BACKEND, std::string("exg_dispatch: ") + hipGetErrorString(e));
+    s->fatals[kind] = fatal;
+    if (fatal)   // mark-and-continue: the outputs are written; the message fails
+        return sfail(EXG_ERR_FATAL, "exg_dispatch: " + std::to_string(fatal) + " row(s) of kind " +
+                                        std::to_string(kind) + " produced non-finite outputs");
     return EXG_OK;
 }
 
@@ -378,7 +408,7 @@ int exg_flush(exg_store* s) {
 
 int exg_fatals(exg_store* s, int kind, uint32_t* count) {
     if (!s || !count || kind < 0 || kind >= exg::N_KINDS) return sfail(EXG_ERR_INVALID, "exg_fatals: bad arguments");
-    *count = 0;  // the f64 kernels carry no guarded-division fatal sites
+    *count = s->fatals[kind];
     return EXG_OK;
 }
 

@@ -33,9 +33,14 @@ KIND = {
 JOINT_SHADER = {0: "SimpleSpringDamper", 1: "CriticallyDampedWarped", 2: "PerpendicularDamperWarped",
                 3: "TorsionalDamperWarped"}
 
+# storage scalar (World::builder().usual::<T>())
+SCALARS = {"f32": 0, "f64": 1}
+
 _u32p = C.POINTER(C.c_uint32)
 _SIGS = {
     "exg_store_create": [C.c_uint64, C.POINTER(C.c_void_p)],
+    "exg_store_create_scalar": [C.c_uint64, C.c_int, C.POINTER(C.c_void_p)],
+    "exg_store_scalar": [C.c_void_p, C.POINTER(C.c_int)],
     "exg_store_destroy": [C.c_void_p],
     "exg_slot_insert": [C.c_void_p, C.c_int, C.c_void_p, _u32p, _u32p],
     "exg_slot_read": [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(C.c_int)],
@@ -69,11 +74,15 @@ def f32_bits(x: float) -> int:
 
 
 class Store:
-    def __init__(self, capacity: int = 256 * 1024):
+    """Slots hold the reference's Pod layouts over the storage scalar: screws as their
+    grade-2 store (see exgame.h), motors as 8 even blades, blocks as [[T; 6]; 6]."""
+
+    def __init__(self, capacity: int = 256 * 1024, scalar: str = "f64"):
         self.L = _bind()
         h = C.c_void_p()
-        _lib.check(self.L.exg_store_create(int(capacity), C.byref(h)))
+        _lib.check(self.L.exg_store_create_scalar(int(capacity), SCALARS[scalar], C.byref(h)))
         self.h = h
+        self.dtype = np.float32 if scalar == "f32" else np.float64
 
     def _buf(self, type_: int, value=None):
         if type_ == ROW:
@@ -82,7 +91,7 @@ class Store:
                 v = np.asarray(value, dtype=np.uint32).reshape(-1)
                 a[: v.size] = v
         else:
-            a = np.zeros(WIDTH[type_], dtype=np.float64)
+            a = np.zeros(WIDTH[type_], dtype=self.dtype)
             if value is not None:
                 a[:] = np.asarray(value, dtype=np.float64).reshape(-1)
         return a
@@ -122,6 +131,12 @@ class Store:
     def dispatch(self, kind: int, rows) -> None:
         slots = np.ascontiguousarray([r.index for r in rows], dtype=np.uint32)
         _lib.check(self.L.exg_dispatch(self.h, int(kind), slots.ctypes.data_as(_u32p), len(slots)))
+
+    def fatals(self, kind: int) -> int:
+        """Rows of the last dispatch of `kind` with non-finite outputs."""
+        out = C.c_uint32()
+        _lib.check(self.L.exg_fatals(self.h, int(kind), C.byref(out)))
+        return out.value
 
     def flush(self) -> None:
         _lib.check(self.L.exg_flush(self.h))

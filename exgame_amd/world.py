@@ -98,6 +98,12 @@ class DeviceWorld:
         _lib.check(_lib.lib().exg_world_stats(self.h, out.ctypes.data_as(C.POINTER(C.c_int32))))
         return out
 
+    def fatals(self) -> int:
+        """Island-steps since creation that ended with a non-finite state."""
+        out = C.c_int64()
+        _lib.check(_lib.lib().exg_world_fatals(self.h, C.byref(out)))
+        return out.value
+
     def last_timing(self) -> dict:
         out = np.zeros(2)
         _lib.check(_lib.lib().exg_world_last_timing(self.h, _lib.fptr(out)))

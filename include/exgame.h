@@ -91,6 +91,10 @@ typedef struct exg_world exg_world;
  *               grouped island by island in declaration order), params[J*10]
  *   G: gravitational constant (gravity off when gravity_on == 0)
  * Returns EXG_ERR_UNSUPPORTED for an island beyond this build's on-chip size limits. */
+/* The state is validated: EXG_ERR_INVALID for a non-positive or non-finite mass, an
+ * inertia tensor that is not finite and positive definite, a non-finite pose,
+ * momentum, size, origin, joint parameter or G; EXG_ERR_UNSUPPORTED for a zero-mass
+ * (kinematic, crates/newton/src/inertia.rs:12-45) body. */
 int exg_world_create(int n_bodies, const double* pose, const double* momentum, const double* mass,
                      const double* inertia, const double* size, int n_islands, const int* island_start,
                      const double* origin, int n_joints, const int* kind, const int* a, const int* b,
@@ -108,7 +112,7 @@ int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm,
 enum { EXG_NS_MFMA = 0, EXG_NS_VALU_ORDERED = 1 };
 int exg_world_set_ns_engine(exg_world* w, int engine);
 
-/* Advance `n_steps` steps of length dt with time warp `warp` (aristotle Epoch dt/warp,
+/* Advance `n_steps` steps of length dt > 0 with time warp `warp` ≥ 1 (aristotle Epoch dt/warp,
  * crates/aristotle/src/epoch.rs:238-248): per step, gravity from the charges published
  * by the previous step (zero on the first, gravity.rs:620-626), the island solve, the
  * re-anchoring and the new charges. Asynchronous on the world's stream. */
@@ -119,8 +123,14 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp);
  * total time (out[0] = -1). No counterpart in the reference. */
 int exg_world_set_graph(exg_world* w, int enable);
 
-/* Block until the world's queued work has finished. */
+/* Block until the world's queued work has finished. Returns EXG_ERR_FATAL when island
+ * steps since the previous sync ended with a non-finite state (mark-and-continue: the
+ * state is kept as computed; exg_world_stats slot 6 counts them per island) — the
+ * analogue of a message failing on its fatal slots (crates/newton/src/accelerator/shaders.rs:411-431). */
 int exg_world_sync(exg_world* w);
+
+/* Island-steps since creation that ended with a non-finite state (synchronizes). */
+int exg_world_fatals(exg_world* w, int64_t* count);
 
 /* Copy the state back: pose[B*8], momentum[B*6], origin[I*3]; any pointer may be NULL. */
 int exg_world_download(exg_world* w, double* pose, double* momentum, double* origin);
@@ -132,7 +142,9 @@ int exg_world_charges(exg_world* w, double* charges);
 int exg_world_external(exg_world* w, double* ext);
 
 /* Per-island counters since creation, I*8: spans, Newton iterations, Newton–Schulz
- * iterations, reseeds, line-search rejections, unconverged commits, 0, 0. */
+ * iterations, reseeds, line-search rejections, unconverged commits, steps that ended
+ * with a non-finite state (fatal), spans found non-finite at the depth limit and not
+ * committed (newton.rs:654-660). */
 int exg_world_stats(exg_world* w, int32_t* stats);
 
 /* Device time of the last exg_world_step call, split by kernel, in milliseconds:
@@ -174,8 +186,26 @@ enum exg_kind {
 
 typedef struct exg_store exg_store;
 
-/* World::builder().capacity(c).usual() — every storage starts with `capacity` slots. */
+/* Scalar type T of a store's storages (World::builder().usual::<T>(); the reference's
+ * device path registers f32, crates/newton/src/accelerator/mod.rs:1015-1019). */
+enum { EXG_SCALAR_F32 = 0, EXG_SCALAR_F64 = 1 };
+
+/* World::builder().capacity(c).usual::<T>() — every storage starts with `capacity`
+ * slots of the reference's Pod layouts over T:
+ *   Motor   8 T  — even blades ascending {1, e12, e13, e23, e01, e02, e03, e0123}
+ *   Twist / Wrench 6 T — the grade-2 store, blades ascending {e12, e13, e23, e01, e02,
+ *           e03} with the markup signs (crates/clifford/src/pga3/screw.rs:102-130; the
+ *           slot/sign map crates/newton/build.rs:106-152 probes): for a screw with
+ *           constructor coordinates (linear|force l, angular|torque a) the store is
+ *           (−a_z, a_y, −a_x, −l_x, −l_y, −l_z)
+ *   [Wrench; 2] 12 T, [[Wrench; 24]; 2] 288 T (offset end·144 + axis·6 + component)
+ *   Block [[T; 6]; 6] in coordinates (row: force|torque, column: linear|angular)
+ *   inertia [T; 3] / [[T; 3]; 3], scalars T, Row [u32; 128].
+ * The kernels compute in f64 and round once when they store. */
+int exg_store_create_scalar(uint64_t capacity, int scalar, exg_store** out);
+/* exg_store_create_scalar(capacity, EXG_SCALAR_F64, out). */
 int exg_store_create(uint64_t capacity, exg_store** out);
+int exg_store_scalar(exg_store* s, int* scalar);
 int exg_store_destroy(exg_store* s);
 /* WriteView::add → RawMap::insert: returns the slot index and its generation. */
 int exg_slot_insert(exg_store* s, int type, const void* value, uint32_t* index, uint32_t* generation);
@@ -191,12 +221,13 @@ int exg_storage_device_ptr(exg_store* s, int type, void** ptr);
  * accelerator/mod.rs:900-964, shaders.rs:369-480). `row_slots` are indices into the
  * ROW storage. Every slot a row names is validated first; two rows writing one slot
  * are refused with EXG_ERR_BACKEND (the Ledger, shaders.rs:287-390). Completes before
- * returning. */
+ * returning. Rows whose outputs are not all finite (a division guard tripped) are
+ * counted and the call returns EXG_ERR_FATAL after writing every output
+ * (mark-and-continue, shaders.rs:411-431). */
 int exg_dispatch(exg_store* s, int kind, const uint32_t* row_slots, uint32_t count);
 /* Wait for queued work (the fence wait of a flush). */
 int exg_flush(exg_store* s);
-/* Fatal-slot count of the last dispatch of `kind` (always 0: the f64 kernels have
- * no guarded-division sites). */
+/* Rows of the last dispatch of `kind` that produced non-finite outputs. */
 int exg_fatals(exg_store* s, int kind, uint32_t* count);
 
 /* Stand-alone pairwise gravity on the device (gravity.rs:110-146, 530-601):
@@ -251,6 +282,7 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
 int exg_arena_retire(exg_arena* r, int world_id);
 /* Advance every admitted world n_steps steps of dt (asynchronous). */
 int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp);
+/* As exg_world_sync, EXG_ERR_FATAL for non-finite island-steps of any world. */
 int exg_arena_sync(exg_arena* r);
 /* One world's state: pose[n*8], momentum[n*6], origin[islands*3]; any may be NULL. */
 int exg_arena_download(exg_arena* r, int world_id, double* pose, double* momentum, double* origin);

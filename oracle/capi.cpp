@@ -69,6 +69,17 @@ void orc_motor_point_raw(const double* m8, const double* p3, double* out16) {
 void orc_twist_transport(const double* t6, const double* m8, double* out6_) { out6(transport_contra(screw6(t6), motor8(m8)), out6_); }
 void orc_wrench_transport(const double* w6, const double* m8, double* out6_) { out6(transport_co(screw6(w6), motor8(m8)), out6_); }
 double orc_power(const double* w6, const double* t6) { return pairing(screw6(w6), screw6(t6)); }
+// A screw's Pod store: its grade-2 blades in ascending order {e12, e13, e23, e01, e02,
+// e03} as Twist::new / Wrench::new write them (screw.rs:240-290), and back.
+void orc_screw_to_pod(const double* c6, double* pod6) {
+    M16 m = screw6(c6);
+    for (int k = 0; k < 6; ++k) pod6[k] = m.c[G2_BLADES[k]];
+}
+void orc_screw_from_pod(const double* pod6, double* c6) {
+    M16 m;
+    for (int k = 0; k < 6; ++k) m.c[G2_BLADES[k]] = pod6[k];
+    out6(m, c6);
+}
 void orc_velocity_at(const double* t6, const double* p3, double* out3) {
     V3<double> v = velocity_at(screw6(t6), make_point(V3<double>(p3[0], p3[1], p3[2])));
     for (int i = 0; i < 3; ++i) out3[i] = v[i];

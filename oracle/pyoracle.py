@@ -94,6 +94,8 @@ def lib():
         "orc_world_front": (None, [C.c_void_p, _dp]),
         "orc_world_stats": (None, [C.c_void_p, _i64p]),
         "orc_world_span_trace": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int32), C.c_int]),
+        "orc_screw_to_pod": (None, [_dp, _dp]),
+        "orc_screw_from_pod": (None, [_dp, _dp]),
         "orc_ns_solve": (None, [C.c_int, _dp, _dp, C.c_int, _dp, _dp, _dp]),
         "orc_world_set_uniform": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
         "orc_rawmap_new": (C.c_void_p, [C.c_uint64]),
@@ -287,6 +289,25 @@ def ns_solve(a_blocks, rhs, iters):
     rho = np.zeros(max(iters, 1))
     lib().orc_ns_solve(m, _p(a), _p(_f64(rhs, m * 6)), int(iters), _p(x), _p(dv), _p(rho))
     return x.reshape(m, m, 6, 6), dv.reshape(m, 6), rho[:iters]
+
+
+def screw_to_pod(c):
+    """The Pod store of screws with constructor coordinates c (..., 6)."""
+    c = np.asarray(c, dtype=np.float64)
+    flat = np.ascontiguousarray(c.reshape(-1, 6))
+    out = np.zeros_like(flat)
+    for i in range(flat.shape[0]):
+        lib().orc_screw_to_pod(_p(flat[i]), _p(out[i]))
+    return out.reshape(c.shape)
+
+
+def screw_from_pod(q):
+    q = np.asarray(q, dtype=np.float64)
+    flat = np.ascontiguousarray(q.reshape(-1, 6))
+    out = np.zeros_like(flat)
+    for i in range(flat.shape[0]):
+        lib().orc_screw_from_pod(_p(flat[i]), _p(out[i]))
+    return out.reshape(q.shape)
 
 
 # ───────────── world ─────────────

@@ -111,7 +111,7 @@ def test_gravity_autotune_keeps_parity():
     w = ar.admit(scene)
     ar.step(1, 1e-2, 1.0)
     chunk = ar.autotune_gravity()
-    assert chunk in (128, 256, 512, 1024, 2048, 4096, 8192)
+    assert chunk in (0, 128, 256, 512, 1024, 2048, 4096, 8192)
     ar.step(2, 1e-2, 1.0)
     orc = pyoracle.OracleWorld(scene, tol2=1e-20)
     orc.step(3, 1e-2, 1.0)
@@ -145,9 +145,47 @@ def test_served_requests_equal_their_alone_runs():
     assert rep["latency_p99_s"] >= rep["latency_p50_s"] > 0
     assert any(r.admitted > r.arrival + 1e-4 for r in reqs)      # queued behind the budget
     for r in reqs[:4]:
-        alone = big_arena(tol2=1e-20)
+        alone = big_arena(tol2=1e-20, gravity_chunk=ar.gravity_chunk)   # the server's chunk setting
         w = alone.admit(r.scene)
         alone.step(r.steps, 1e-2, 1.0)
         a = alone.state(w)
         for k in ("pose", "momentum", "origin"):
             assert np.array_equal(a[k], r.result[k]), (r.rid, k)
+
+
+def same_state(a, b):
+    return all(np.array_equal(a[k], b[k]) for k in ("pose", "momentum", "origin"))
+
+
+def test_arena_equals_single_world_path_beyond_one_chunk():
+    # 3,000 bodies: the single-world path splits the sources over many chunks; an arena
+    # world admitted under the default setting is chunked the same way
+    scene = scenes.mixed_world(3000, seed=31)
+    ar = big_arena(tol2=1e-20)
+    wid = ar.admit(scene)
+    dev = DeviceWorld(scene, tol2=1e-20)
+    ar.step(3, 1e-2, 1.0)
+    dev.step(3, 1e-2, 1.0)
+    assert same_state(ar.state(wid), dev.state())
+
+
+def test_autotune_changes_only_later_admissions():
+    w1, w2, w3 = scenes.mixed_world(2000, seed=41), scenes.mixed_world(1500, seed=42), scenes.mixed_world(1800, seed=43)
+    mix = big_arena(tol2=1e-20)
+    a1 = mix.admit(w1)
+    mix.admit(w2)
+    mix.step(1, 1e-2, 1.0)
+    chunk = mix.autotune_gravity()
+    a3 = mix.admit(w3)
+    mix.step(3, 1e-2, 1.0)
+    # the world admitted before the autotune keeps its chunking: as if alone, untuned
+    alone1 = big_arena(tol2=1e-20)
+    b1 = alone1.admit(w1)
+    alone1.step(4, 1e-2, 1.0)
+    assert same_state(mix.state(a1), alone1.state(b1))
+    # the world admitted after it takes the tuned chunking: as if alone at that setting
+    alone3 = big_arena(tol2=1e-20)
+    alone3.set_gravity_chunk(chunk)
+    b3 = alone3.admit(w3)
+    alone3.step(3, 1e-2, 1.0)
+    assert same_state(mix.state(a3), alone3.state(b3))

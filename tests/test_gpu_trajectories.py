@@ -188,3 +188,35 @@ def test_a_stale_warm_hint_is_reseeded_not_followed():
     assert np.all(np.isfinite(a))
     assert rel(a, orc.state()["position"]) < 1e-9
     assert dev.stats()[0, 3] > 0
+
+
+def coincident_pair():
+    # two bodies at one point joined by an axial spring with zero softening: the
+    # spring's direction divides by a zero distance
+    import dataclasses
+    sc = spring_pair((0, 0, 0), (0, 0, 0), rest=1.0, k=10.0, c=0.0)
+    return dataclasses.replace(sc, joint_params=np.array([[0, 0, 0, 0, 0, 0, 1.0, 10.0, 0.0, 0.0]]))
+
+
+def test_non_finite_explicit_step_is_reported_fatal():
+    from exgame_amd import _lib
+    dev = DeviceWorld(coincident_pair(), integrator="symplectic_euler", gravity=False)
+    dev.step(1, 1e-2, 1.0)
+    with pytest.raises(_lib.ExgError) as e:
+        dev.sync()
+    assert e.value.code == 3                      # EXG_ERR_FATAL, mark-and-continue
+    assert dev.stats()[0, 6] == 1 and dev.fatals() == 1
+    dev.sync()                                    # reported once
+    assert not np.all(np.isfinite(dev.state()["momentum"]))
+
+
+def test_non_finite_implicit_spans_are_not_committed():
+    # Newton never commits a non-finite span (newton.rs:654-660): the state stays as it
+    # was, every span at the depth limit is counted, and no step is fatal
+    sc = coincident_pair()
+    dev = DeviceWorld(sc, gravity=False)
+    dev.step(1, 1e-2, 1.0)
+    dev.sync()
+    st = dev.stats()[0]
+    assert st[6] == 0 and st[7] == 256
+    assert np.array_equal(dev.state()["pose"], sc.pose)
