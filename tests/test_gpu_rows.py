@@ -250,17 +250,16 @@ def test_dispatch_refuses_bad_rows_and_two_writers():
 
 
 def test_fatal_rows_are_marked_and_their_outputs_kept():
-    # an axial spring with zero softening between coincident anchors divides by a zero
-    # distance: the row is marked fatal (EXG_ERR_FATAL), the healthy row is written
+    # a connection row whose velocity slot holds a NaN yields non-finite wrenches: the
+    # row is marked fatal (EXG_ERR_FATAL), every row's outputs are written
     st = S.Store(capacity=32)
     acc = S.Accelerator(st)
     dt, warp = st.insert(S.SCALAR, 0.01), st.insert(S.SCALAR, 1.0)
-    pose = rmotor()
-    good_p, bad_p = [0.3, 0, 0, -0.3, 0, 0, 1.0, 5.0, 1.0, 1e-6], [0, 0, 0, 0, 0, 0, 1.0, 5.0, 1.0, 0.0]
+    prm = [0.3, 0, 0, -0.3, 0, 0, 1.0, 5.0, 1.0, 1e-6]
     rows, outs = [], []
-    for prm, pb in ((good_p, rmotor()), (bad_p, pose)):
-        keys = [st.insert(S.TWIST, np.zeros(6)), st.insert(S.TWIST, np.zeros(6)), st.insert(S.MOTOR, pose),
-                st.insert(S.MOTOR, pb)]
+    for vel in (RNG.normal(size=6), np.full(6, np.nan)):
+        keys = [st.insert(S.TWIST, POD(vel)), st.insert(S.TWIST, np.zeros(6)), st.insert(S.MOTOR, rmotor()),
+                st.insert(S.MOTOR, rmotor())]
         pk = [st.insert(S.SCALAR, p).index for p in prm]
         outs.append(st.insert(S.WRENCH_PAIR, np.zeros(12)))
         rows.append(row(st, [k.index for k in keys] + [outs[-1].index, dt.index, warp.index] + pk))
