@@ -241,6 +241,19 @@ int exg_gravity_accumulate(int n, const double* charges, double G, double* wrenc
  * state update stay f64 either way. */
 enum { EXG_PREC_F64 = 0, EXG_PREC_F32_COMPENSATED = 1 };
 int exg_gravity_accumulate_precision(int n, const double* charges, double G, int precision, double* wrench);
+/* Pairwise gravity that allocates no device memory — no counterpart in the reference;
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The same arithmetic and work split as
+ * exg_gravity_accumulate_precision (results bit-identical), but `charges` (n*8) and
+ * `wrench` (n*6) are DEVICE buffers the caller owns, and the work table and partial
+ * sums live in a fixed device array of the library (EXG_GRAVITY_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends). n ≤ EXG_GRAVITY_NOALLOC_MAX_BODIES, else
+ * EXG_ERR_INVALID. Asynchronous on `stream` (a hipStream_t; NULL for the default
+ * stream). Calls that overlap on one GPU share the array and race: serialise them. */
+#define EXG_GRAVITY_NOALLOC_MAX_BODIES 65536
+#define EXG_GRAVITY_NOALLOC_SCRATCH_BYTES 25452544
+int exg_gravity_accumulate_noalloc(int n, const double* charges, double G, int precision, double* wrench, void* stream);
+
 /* Per-world precision of gravity and of the Jacobian assembly: with
  * EXG_PREC_F32_COMPENSATED the joint Jacobians are evaluated in FP32 forward mode and
  * assembled in FP32 with two-sum accumulation; the residual's wrenches, Newton–Schulz
