@@ -98,7 +98,7 @@ struct exg_arena {
     double tol2 = 1e-8;
     int ns_warm = 2, ns_seed = 24, ns_engine = EXG_NS_MFMA, grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     hipStream_t stream = nullptr;
-    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G;
+    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G, grav_stage;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;                // island-steps that ended with a non-finite state
     long long fatal_reported = 0;
@@ -125,7 +125,7 @@ struct exg_arena {
 
     void release() {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
-                        &partial, &island_G})
+                        &partial, &island_G, &grav_stage})
             b->release();
         for (auto* b : {&island_start, &island_end, &island_jstart, &island_jend, &jkind, &ja, &jb, &seeded, &stats,
                         &fatal})
@@ -245,6 +245,7 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
     R_TRY(r->ext.alloc(B * 6));
     R_TRY(r->charges[0].alloc(B * 8));
     R_TRY(r->charges[1].alloc(B * 8));
+    R_TRY(r->grav_stage.alloc(B * exg::GRAV_STAGE_BYTES / sizeof(double)));
     R_TRY(r->origin.alloc(I * 3));
     R_TRY(r->island_G.alloc(I));
     R_TRY(r->island_start.alloc(I));
@@ -453,7 +454,7 @@ static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
     double* front = r->charges[r->front].p;
     double* back = r->charges[1 - r->front].p;
     exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                        r->partial.p, r->ext.p, r->stream);
+                        r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
     A_TRY(hipGetLastError());
     exg::IslandArgs a{};
     a.pose = r->pose.p;
@@ -571,11 +572,11 @@ int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
         if (int rc = r->rebuild(c)) return rc;
         double* front = r->charges[r->front].p;
         exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                            r->partial.p, r->ext.p, r->stream);
+                            r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
         A_TRY(hipEventRecord(r->ev[0], r->stream));
         for (int k = 0; k < 5; ++k)
             exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                                r->partial.p, r->ext.p, r->stream);
+                                r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
         A_TRY(hipEventRecord(r->ev[1], r->stream));
         A_TRY(hipGetLastError());
         A_TRY(hipEventSynchronize(r->ev[1]));

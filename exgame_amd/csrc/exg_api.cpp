@@ -129,7 +129,7 @@ struct exg_world {
     DevBuf<int> class_ids[exg::ISLAND_CLASSES];   // island ids routed to each kernel variant
     int class_n[exg::ISLAND_CLASSES] = {};
     hipStream_t stream = nullptr;
-    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial;
+    DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial, grav_stage;
     DevBuf<exg::GravEntry> grav_entries;
     DevBuf<exg::GravTile> grav_tiles;
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
@@ -153,7 +153,7 @@ struct exg_world {
 
     void release() {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
-                        &grav_partial})
+                        &grav_partial, &grav_stage})
             b->release();
         for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats, &fatal}) b->release();
         grav_entries.release();
@@ -253,6 +253,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
         W_TRY(w->grav_tiles.alloc(tiles.size()));
         W_TRY(w->grav_tiles.upload(tiles.data(), tiles.size()));
         W_TRY(w->grav_partial.alloc(entries.size() * exg::GRAV_PARTIAL_PER_ENTRY));
+        W_TRY(w->grav_stage.alloc(B * exg::GRAV_STAGE_BYTES / sizeof(double)));
     }
     W_TRY(w->island_start.alloc(I + 1));
     W_TRY(w->island_jstart.alloc(I + 1));
@@ -370,7 +371,7 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     hipEvent_t g0 = ev ? ev[0] : nullptr, g1 = ev ? ev[1] : nullptr;
     if (w->gravity) {
         exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p, (int)w->grav_tiles.n,
-                            w->grav_prec, w->grav_partial.p, w->ext.p, w->stream, g0, g1);
+                            w->grav_prec, w->grav_partial.p, w->ext.p, w->grav_stage.p, w->B, w->stream, g0, g1);
     } else {
         exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream, g0, g1);
     }
@@ -617,11 +618,12 @@ int exg_gravity_accumulate_precision(int n, const double* charges, double G, int
     std::vector<exg::GravEntry> entries;
     std::vector<exg::GravTile> tiles;
     exg::gravity_add_world(entries, tiles, 0, n, G, cus);
-    DevBuf<double> dc, dw, dp;
+    DevBuf<double> dc, dw, dp, ds;
     DevBuf<exg::GravEntry> de;
     DevBuf<exg::GravTile> dt;
     hipError_t e = dc.alloc((size_t)n * 8);
     if (e == hipSuccess) e = dw.alloc((size_t)n * 6);
+    if (e == hipSuccess) e = ds.alloc((size_t)n * exg::GRAV_STAGE_BYTES / sizeof(double));
     if (e == hipSuccess) e = dp.alloc(entries.size() * exg::GRAV_PARTIAL_PER_ENTRY);
     if (e == hipSuccess) e = de.alloc(entries.size());
     if (e == hipSuccess) e = dt.alloc(tiles.size());
@@ -629,11 +631,12 @@ int exg_gravity_accumulate_precision(int n, const double* charges, double G, int
     if (e == hipSuccess) e = de.upload(entries.data(), entries.size());
     if (e == hipSuccess) e = dt.upload(tiles.data(), tiles.size());
     if (e == hipSuccess) {
-        exg::launch_gravity(dc.p, de.p, (int)entries.size(), dt.p, (int)tiles.size(), precision, dp.p, dw.p, nullptr);
+        exg::launch_gravity(dc.p, de.p, (int)entries.size(), dt.p, (int)tiles.size(), precision, dp.p, dw.p, ds.p, n,
+                            nullptr);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpy(wrench, dw.p, (size_t)n * 6 * sizeof(double), hipMemcpyDeviceToHost);
-    for (auto* b : {&dc, &dw, &dp}) b->release();
+    for (auto* b : {&dc, &dw, &dp, &ds}) b->release();
     de.release();
     dt.release();
     if (e != hipSuccess) return fail(EXG_ERR_BACKEND, std::string("exg_gravity_accumulate: ") + hipGetErrorString(e));

@@ -26,11 +26,10 @@
 // f64 pairs: one hardware reciprocal square root refined by a third-order step
 // replaces the reference's sqrt and three divisions; the result is within 2 ulp of
 // the correctly rounded form.
-// f32c pairs (mixed precision): sources and target are narrowed to f32 as they are
-// staged; a pair costs one v_rsq_f32 and a dozen f32 operations, and each force
-// component is accumulated as an unevaluated sum hi + lo — after every addition the
-// rounding error of hi + x is recovered exactly (Knuth's two-sum) and added to lo.
-// The slice sums are widened to f64 for the reductions, scaling and moment.
+// f32 pairs (mixed precision): positions are staged as double-single hi + lo pairs of
+// f32 (see f32ds_block), a pair costs one v_rsq_f32 and about fifteen f32 operations,
+// and the f32 sums of groups of 16 sources are added into f64 accumulators; the
+// reductions, scaling and moment are f64.
 #include <hip/hip_runtime.h>
 #include "island.h"
 
@@ -48,14 +47,6 @@ __device__ inline double rsqrt_refined(double x) {
     const double e = fma(-(x * y0), y0, 1.0);
     const double c = fma(0.375, e, 0.5);
     return fma(y0 * e, c, y0);
-}
-
-__device__ inline void two_sum_acc(float& hi, float& lo, float x) {
-    const float t = hi + x;
-    const float bp = t - hi;
-    const float e = (hi - (t - bp)) + (x - bp);
-    hi = t;
-    lo = lo + e;
 }
 
 // Wave-local staging: a wave's LDS rows are written and read by that wave only.
@@ -97,32 +88,21 @@ __device__ __forceinline__ void f64_block(const double2* sxy, const double2* szg
     if (q < cnt) pair(q, f[0]);
 }
 
-template <int MODE>
-__device__ __forceinline__ void f32c_block(const float4* sp, const float* sc, int cnt, int base, int i, float4 me,
-                                           float minv, float hi[2][3], float lo[2][3]) {
-    auto pair = [&](int q, float* h3, float* l3) {
-        const int jj = base + q;
-        const float4 o = sp[q];
-        const float dx = o.x - me.x, dy = o.y - me.y, dz = o.z - me.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        float inv_s = minv;
-        if (MODE == 1) inv_s = sc[q];
-        if (MODE == 2) inv_s = jj > i ? sc[q] : minv;
-        const float inv_r = fminf(__builtin_amdgcn_rsqf(d2), inv_s);
-        float w = o.w * (inv_r * inv_r * inv_r);
-        w = ((MODE == 2 && jj == i) || d2 < 1e-18f) ? 0.f : w;
-        two_sum_acc(h3[0], l3[0], w * dx);
-        two_sum_acc(h3[1], l3[1], w * dy);
-        two_sum_acc(h3[2], l3[2], w * dz);
-    };
-    int q = 0;
-    for (; q + 2 <= cnt; q += 2) {
-        pair(q, hi[0], lo[0]);
-        pair(q + 1, hi[1], lo[1]);
-    }
-    if (q < cnt) pair(q, hi[0], lo[0]);
+// f32 pairs on double-single positions: each coordinate is carried as hi + lo, two
+// f32 values with hi = f32(x) and lo = f32(x − hi), so a difference is
+// (x_hi − y_hi) + (x_lo − y_lo) — exact in its leading part for close pairs (the hi
+// parts agree in their leading bits), rounded once — instead of the difference of
+// two positions rounded to f32 (which loses the bits below 2^-24·|x|, a relative error
+// of |x|/d·6e-8 on a pair at distance d). 1/d is the hardware estimate refined by one
+// Newton step (its error enters 1/d³ three times over), the core law is
+// min(1/d, 1/s), and the force is summed in f32 over GROUP sources at a time (two
+// interleaved sums), each group's sum added to f64 accumulators.
+constexpr int F32_GROUP = 16;
+__device__ inline float rsqrt_refined_f32(float x) {
+    const float y = __builtin_amdgcn_rsqf(x);
+    const float e = fmaf(-(x * y), y, 1.0f);
+    return fmaf(y * e, 0.5f, y);
 }
-
 // slice sums added in slice order; wave 0 writes the entry's partial
 __device__ inline void write_partial(double fx, double fy, double fz, bool live, double* __restrict__ partial) {
     __shared__ double red[3][GRAV_WAVES - 1][GRAV_TILE];
@@ -152,7 +132,8 @@ __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f64_kernel(co
     __shared__ double2 sxy[GRAV_WAVES][64], szg[GRAV_WAVES][64];
     __shared__ double sinv[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the wave's index is wave-uniform: its slice bounds and loop counts live in scalar registers
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int i = en.tgt0 + lane;
     const bool live = lane < en.ntgt;
     double xi = 0.0, yi = 0.0, zi = 0.0, sinv_i = 1.0;
@@ -184,42 +165,97 @@ __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f64_kernel(co
     write_partial(f[0][0] + f[1][0], f[0][1] + f[1][1], f[0][2] + f[1][2], live, partial);
 }
 
+// hi + lo split of a coordinate (lo = f32 of the remainder)
+__device__ inline void split_ds(double x, float& hi, float& lo) {
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+}
+
+// The f32 source records of bodies [0, n): x_hi y_hi z_hi G·m | x_lo y_lo z_lo 1/s.
+__global__ __launch_bounds__(256) void gravity_stage_f32_kernel(const double* __restrict__ ch, int n,
+                                                                float4* __restrict__ stage) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const double* c = ch + (size_t)j * 8;
+    float4 h, l;
+    split_ds(c[0], h.x, l.x);
+    split_ds(c[1], h.y, l.y);
+    split_ds(c[2], h.z, l.z);
+    h.w = (float)c[6];
+    l.w = (float)(1.0 / c[7]);
+    stage[2 * (size_t)j] = h;
+    stage[2 * (size_t)j + 1] = l;
+}
+
+// Sources are read from the staged records with scalar loads (every lane of a wave
+// reads the same source), so each pair's source operands sit in scalar registers.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) f4v* ConstF4;
+
+template <int MODE>
+__device__ __forceinline__ void f32ds_span(ConstF4 src, int j0, int cnt, int i, const float4& mh, const float4& ml,
+                                           float minv, double f[3]) {
+    int q = 0;
+    while (q < cnt) {
+        const int qe = min(cnt, q + F32_GROUP);
+        float a[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        auto pair = [&](int k, float* acc) {
+            const int jj = j0 + k;
+            const f4v h = src[2 * jj], l = src[2 * jj + 1];
+            const float dx = (h.x - mh.x) + (l.x - ml.x);
+            const float dy = (h.y - mh.y) + (l.y - ml.y);
+            const float dz = (h.z - mh.z) + (l.z - ml.z);
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            float inv_s = minv;
+            if (MODE == 1) inv_s = l.w;
+            if (MODE == 2) inv_s = jj > i ? l.w : minv;
+            const float inv_r = fminf(rsqrt_refined_f32(d2), inv_s);
+            float w = h.w * (inv_r * inv_r * inv_r);
+            w = ((MODE == 2 && jj == i) || d2 < 1e-18f) ? 0.f : w;
+            acc[0] = fmaf(w, dx, acc[0]);
+            acc[1] = fmaf(w, dy, acc[1]);
+            acc[2] = fmaf(w, dz, acc[2]);
+        };
+        int k = q;
+#pragma unroll 4
+        for (; k + 2 <= qe; k += 2) {
+            pair(k, a[0]);
+            pair(k + 1, a[1]);
+        }
+        if (k < qe) pair(k, a[0]);
+        for (int c = 0; c < 3; ++c) f[c] += (double)a[0][c] + (double)a[1][c];
+        q = qe;
+    }
+}
+
 __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f32c_kernel(const double* __restrict__ ch,
                                                                                const GravEntry* __restrict__ entries,
+                                                                               const float4* __restrict__ stage,
                                                                                double* __restrict__ partial) {
-    __shared__ float4 sp[GRAV_WAVES][64];
-    __shared__ float sc[GRAV_WAVES][64];
     const GravEntry en = entries[blockIdx.x];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the wave's index is wave-uniform: its slice bounds and loop counts live in scalar registers
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int i = en.tgt0 + lane;
     const bool live = lane < en.ntgt;
-    float4 me = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 mh = make_float4(0.f, 0.f, 0.f, 0.f), ml = make_float4(0.f, 0.f, 0.f, 0.f);
     float minv = 1.f;
     if (live) {
-        const double* c = ch + (size_t)i * 8;
-        me = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-        minv = 1.0f / (float)c[7];
+        const float4 h = stage[2 * (size_t)i], l = stage[2 * (size_t)i + 1];
+        mh = h;
+        ml = l;
+        minv = l.w;
     }
     const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
     const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
     const int t_lo = en.tgt0, t_hi = en.tgt0 + en.ntgt - 1;
-    float hi[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}}, lo[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    ConstF4 src = (ConstF4)stage;
+    double f[3] = {0.0, 0.0, 0.0};
     for (int base = s0; base < s1; base += 64) {
-        const int j = base + lane;
-        if (j < s1) {
-            const double* c = ch + (size_t)j * 8;
-            sp[w][lane] = make_float4((float)c[0], (float)c[1], (float)c[2], (float)c[6]);
-            sc[w][lane] = 1.0f / (float)c[7];
-        }
-        wave_lds_sync();
         const int cnt = min(64, s1 - base);
-        if (base + cnt - 1 < t_lo) f32c_block<0>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
-        else if (base > t_hi) f32c_block<1>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
-        else f32c_block<2>(sp[w], sc[w], cnt, base, i, me, minv, hi, lo);
-        wave_lds_sync();
+        if (base + cnt - 1 < t_lo) f32ds_span<0>(src, base, cnt, i, mh, ml, minv, f);
+        else if (base > t_hi) f32ds_span<1>(src, base, cnt, i, mh, ml, minv, f);
+        else f32ds_span<2>(src, base, cnt, i, mh, ml, minv, f);
     }
-    double f[3];
-    for (int k = 0; k < 3; ++k) f[k] = ((double)hi[0][k] + (double)lo[0][k]) + ((double)hi[1][k] + (double)lo[1][k]);
     write_partial(f[0], f[1], f[2], live, partial);
 }
 
@@ -319,8 +355,8 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
 }
 
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
-                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream, hipEvent_t start,
-                    hipEvent_t stop) {
+                    int n_tiles, int precision, double* partial, double* ext, void* stage, int n_bodies,
+                    hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
     if (n_tiles <= 0) return;
     // tiles without entries (worlds without gravity) still get their zero wrench from
     // the reduce, also when no world of the launch has gravity
@@ -329,10 +365,12 @@ void launch_gravity(const double* charges, const GravEntry* entries, int n_entri
                  (const double*)partial, ext);
         return;
     }
-    if (precision == 1)
-        launch_k(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
-                 entries, partial);
-    else
+    if (precision == 1) {
+        launch_k(gravity_stage_f32_kernel, dim3((n_bodies + 255) / 256), dim3(256), stream, start, nullptr, charges,
+                 n_bodies, (float4*)stage);
+        launch_k(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, nullptr, nullptr, charges,
+                 entries, (const float4*)stage, partial);
+    } else
         launch_k(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
                  entries, partial);
     launch_k(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), stream, nullptr, stop, charges, tiles,

@@ -140,10 +140,13 @@ int gravity_chunk_len(int n, int compute_units);
 // The same with a fixed chunk length (≥ 64 sources).
 void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                                int chunk_len);
-// precision 0: f64 pairs; 1: f32 pairs with compensated accumulation
+// precision 0: f64 pairs; 1: f32 pairs on double-single positions, f64 accumulation.
+// `stage` holds GRAV_STAGE_BYTES per body for bodies [0, n_bodies) (every body an
+// entry names): the per-source records the f32 kernel reads.
+constexpr int GRAV_STAGE_BYTES = 64;
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
-                    int n_tiles, int precision, double* partial, double* ext, hipStream_t stream, hipEvent_t start = nullptr,
-                    hipEvent_t stop = nullptr);
+                    int n_tiles, int precision, double* partial, double* ext, void* stage, int n_bodies,
+                    hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream, hipEvent_t start = nullptr,
                         hipEvent_t stop = nullptr);
 

@@ -4,7 +4,8 @@
 // It computes what exg_gravity_accumulate_precision computes, with the same work
 // split and kernels, so the two agree bit for bit; but the caller passes device
 // buffers for the charges and the wrenches, and the work table and the per-chunk
-// partial sums live in `scratch`, a fixed device array of this library (one copy
+// partial sums (and the per-source records of the f32 kernel) live in `scratch`, a
+// fixed device array of this library (one copy
 // per GPU, held until the process ends). The array's device address is taken once
 // per device with hipGetSymbolAddress and only that address is ever handed to
 // kernels. Every launch rewrites the parts of `scratch` it reads (the table fill
@@ -26,11 +27,13 @@ constexpr int NOALLOC_MAX_ENTRIES = 16384;   // the work split's cap: 64 workgro
 constexpr size_t PARTIAL_BYTES = (size_t)NOALLOC_MAX_ENTRIES * GRAV_PARTIAL_PER_ENTRY * sizeof(double);
 constexpr size_t ENTRY_BYTES = (size_t)NOALLOC_MAX_ENTRIES * sizeof(GravEntry);
 constexpr size_t TILE_BYTES = (size_t)NOALLOC_MAX_TILES * sizeof(GravTile);
-constexpr size_t SCRATCH_BYTES = PARTIAL_BYTES + ENTRY_BYTES + TILE_BYTES;
-static_assert(PARTIAL_BYTES % 256 == 0 && ENTRY_BYTES % 256 == 0, "scratch sections stay 256-byte aligned");
+constexpr size_t STAGE_BYTES = (size_t)NOALLOC_MAX_BODIES * GRAV_STAGE_BYTES;
+constexpr size_t SCRATCH_BYTES = PARTIAL_BYTES + STAGE_BYTES + ENTRY_BYTES + TILE_BYTES;
+static_assert(PARTIAL_BYTES % 256 == 0 && STAGE_BYTES % 256 == 0 && ENTRY_BYTES % 256 == 0,
+              "scratch sections stay 256-byte aligned");
 static_assert(SCRATCH_BYTES == EXG_GRAVITY_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
 
-// partial sums | work entries | work tiles
+// partial sums | per-source records | work entries | work tiles
 static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 
 constexpr int MAX_DEVICES = 64;
@@ -111,13 +114,14 @@ extern "C" int exg_gravity_accumulate_noalloc(int n, const double* charges, doub
         return set_error(EXG_ERR_UNSUPPORTED, "exg_gravity_accumulate_noalloc: the work split exceeds the scratch table "
                                               "(a device with more than 256 compute units)");
     double* partial = reinterpret_cast<double*>(base);
-    GravEntry* entries = reinterpret_cast<GravEntry*>(base + PARTIAL_BYTES);
-    GravTile* tiles = reinterpret_cast<GravTile*>(base + PARTIAL_BYTES + ENTRY_BYTES);
+    void* stage = base + PARTIAL_BYTES;
+    GravEntry* entries = reinterpret_cast<GravEntry*>(base + PARTIAL_BYTES + STAGE_BYTES);
+    GravTile* tiles = reinterpret_cast<GravTile*>(base + PARTIAL_BYTES + STAGE_BYTES + ENTRY_BYTES);
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int n_entries = ntiles * chunks;
     hipLaunchKernelGGL(noalloc_table_kernel, dim3((n_entries + 255) / 256), dim3(256), 0, st, entries, tiles, n, chunk,
                        chunks, 1.0 / G);
-    launch_gravity(charges, entries, n_entries, tiles, ntiles, precision, partial, wrench, st);
+    launch_gravity(charges, entries, n_entries, tiles, ntiles, precision, partial, wrench, stage, n, st);
     e = hipGetLastError();
     if (e != hipSuccess)
         return set_error(EXG_ERR_BACKEND, (std::string("exg_gravity_accumulate_noalloc: ") + hipGetErrorString(e)).c_str());

@@ -251,7 +251,7 @@ int exg_gravity_accumulate_precision(int n, const double* charges, double G, int
  * EXG_ERR_INVALID. Asynchronous on `stream` (a hipStream_t; NULL for the default
  * stream). Calls that overlap on one GPU share the array and race: serialise them. */
 #define EXG_GRAVITY_NOALLOC_MAX_BODIES 65536
-#define EXG_GRAVITY_NOALLOC_SCRATCH_BYTES 25452544
+#define EXG_GRAVITY_NOALLOC_SCRATCH_BYTES 29646848
 int exg_gravity_accumulate_noalloc(int n, const double* charges, double G, int precision, double* wrench, void* stream);
 
 /* Per-world precision of gravity and of the Jacobian assembly: with
