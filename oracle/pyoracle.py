@@ -29,9 +29,23 @@ _u32p = C.POINTER(C.c_uint32)
 _u64p = C.POINTER(C.c_uint64)
 
 
+NATIVE_LIB_PATH = os.path.join(HERE, "_build", "liborc_native.so")
+
+
 def build() -> None:
     """Compile the oracle with its own Makefile (gcc, no GPU)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def use_native() -> None:
+    """Compile the oracle for this host's CPU (-march=native: the CPU-baseline timing
+    build; always rebuilt, a copy made on another machine may not run here) and load
+    it. Call before any other use."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("use_native() must come before the oracle is loaded")
+    subprocess.run(["make", "-s", "-B", "-C", HERE, "native"], check=True)
+    LIB_PATH = NATIVE_LIB_PATH
 
 
 def _p(a: np.ndarray):
