@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libexgame_hip.so")
 
-SOURCES = ["island.hip", "singles.hip", "explicit.hip", "gravity.hip", "rows.hip", "exg_api.cpp", "store.cpp", "topology.cpp",
+SOURCES = ["island.hip", "island_large.hip", "singles.hip", "explicit.hip", "gravity.hip", "rows.hip", "exg_api.cpp", "store.cpp", "topology.cpp",
            "arena.cpp", "noalloc.hip"]
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # -packed-fp32-ops: no packed FP32 instructions anywhere in this library.

@@ -336,6 +336,9 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
         const int j0 = jstart[i], ib0 = island_start[i];
+        if (exg::island_class(island_start[i + 1] - ib0, jstart[i + 1] - j0) == exg::ISLAND_CLASS_LARGE)
+            return afail(EXG_ERR_UNSUPPORTED, "exg_arena_admit: arenas take islands of up to 8 bodies and 16 joints "
+                                              "(larger ones run in a world of their own)");
         if (exg::island_topology_of(island_start[i + 1] - ib0, jstart[i + 1] - j0, kind + j0, a + j0, b + j0, ib0,
                                     topo[i]) != 0)
             return afail(EXG_ERR_UNSUPPORTED, "exg_arena_admit: an island exceeds the kernels' topology record");

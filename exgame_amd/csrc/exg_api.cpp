@@ -29,8 +29,10 @@ int fail(int code, const std::string& msg) {
 
 using exg::DevBuf;
 
-constexpr int MAX_ISLAND_BODIES = 8;
-constexpr int MAX_ISLAND_JOINTS = 16;
+constexpr int MAX_ISLAND_BODIES = exg::LARGE_MAXM;
+constexpr int MAX_ISLAND_JOINTS = exg::LARGE_MAXJ;
+constexpr int EXPLICIT_MAX_BODIES = 8;    // the explicit kernel keeps an island in registers
+constexpr int EXPLICIT_MAX_JOINTS = 16;
 
 }  // namespace
 
@@ -68,8 +70,10 @@ int exg::check_world(const char* who, int n_bodies, int n_islands, const int* is
         jstart[i + 1] += jstart[i];
     }
     if (max_m > MAX_ISLAND_BODIES || max_j > MAX_ISLAND_JOINTS)
-        return fail(EXG_ERR_UNSUPPORTED, std::string(who) + ": islands up to 8 bodies and 16 joints are supported (got " +
-                                             std::to_string(max_m) + " bodies, " + std::to_string(max_j) + " joints)");
+        return fail(EXG_ERR_UNSUPPORTED, std::string(who) + ": islands up to " + std::to_string(MAX_ISLAND_BODIES) +
+                                             " bodies and " + std::to_string(MAX_ISLAND_JOINTS) +
+                                             " joints are supported (got " + std::to_string(max_m) + " bodies, " +
+                                             std::to_string(max_j) + " joints)");
 
     return EXG_OK;
 }
@@ -138,6 +142,9 @@ struct exg_world {
     long long fatal_reported = 0;   // the count the last exg_world_sync reported
     DevBuf<long long> xacc_off;
     DevBuf<exg::IslandTopo> topo;   // per island, Newton integrator only
+    DevBuf<exg::LargeTopo> ltopo;   // per island when the world has large islands (others: unused)
+    DevBuf<double> lws;             // the large islands' workspace
+    DevBuf<long long> lws_off;      // per island
     int front = 0;
     std::vector<hipEvent_t> ev;
     int ev_used = 0;
@@ -160,6 +167,9 @@ struct exg_world {
         grav_tiles.release();
         xacc_off.release();
         topo.release();
+        ltopo.release();
+        lws.release();
+        lws_off.release();
         for (auto& c : class_ids) c.release();
         for (auto e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -300,6 +310,36 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     W_TRY(up(w->xacc_off.p, xoff.data(), I * sizeof(long long)));
     W_TRY(w->topo.alloc(I));
     W_TRY(w->topo.upload(topo.data(), I));
+    if (w->class_n[exg::ISLAND_CLASS_LARGE]) {
+        // large islands: a topology record and a workspace each (records of the other
+        // islands stay zero and are never read)
+        std::vector<exg::LargeTopo> lt(I);
+        std::vector<long long> loff(I, 0);
+        long long ltot = 0;
+        for (size_t i = 0; i < I; ++i) {
+            const int mm = island_start[i + 1] - island_start[i], jj = jstart[i + 1] - jstart[i];
+            if (exg::island_class(mm, jj) != exg::ISLAND_CLASS_LARGE) {
+                std::memset(&lt[i], 0, sizeof(exg::LargeTopo));
+                continue;
+            }
+            std::vector<int> la(jj), lb(jj);
+            for (int q = 0; q < jj; ++q) {
+                la[q] = a[jstart[i] + q] - island_start[i];
+                lb[q] = b[jstart[i] + q] - island_start[i];
+            }
+            if (exg::build_large_topology(mm, jj, la.data(), lb.data(), lt[i]) != 0) {
+                fail(EXG_ERR_UNSUPPORTED, "exg_world_create: island " + std::to_string(i) + " exceeds the large-island record");
+                return bail(EXG_ERR_UNSUPPORTED);
+            }
+            loff[i] = ltot;
+            ltot += exg::large_workspace_doubles(mm, jj);
+        }
+        W_TRY(w->ltopo.alloc(I));
+        W_TRY(w->ltopo.upload(lt.data(), I));
+        W_TRY(w->lws.alloc((size_t)ltot));
+        W_TRY(w->lws_off.alloc(I));
+        W_TRY(w->lws_off.upload(loff.data(), I));
+    }
     if (Jn) {
         W_TRY(up(w->jkind.p, kind, Jn * sizeof(int)));
         W_TRY(up(w->ja.p, a, Jn * sizeof(int)));
@@ -397,6 +437,9 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.jb = w->jb.p;
     a.jparams = w->jparams.p;
     a.topo = w->topo.p;
+    a.ltopo = w->ltopo.p;
+    a.lws = w->lws.p;
+    a.lws_off = w->lws_off.p;
     a.xacc = w->xacc.p;
     a.xacc_off = w->xacc_off.p;
     a.seeded = w->seeded.p;
@@ -478,6 +521,9 @@ int exg_world_step(exg_world* w, int n_steps, double dt, double warp) {
     // the engine's input facts (crates/newton/build.rs input_facts): dt > 0, warp ≥ 1
     if (!(dt > 0.0) || !std::isfinite(dt) || !(warp >= 1.0) || !std::isfinite(warp))
         return fail(EXG_ERR_INVALID, "exg_world_step: dt must be positive and warp at least 1 (both finite)");
+    if (w->integrator != EXG_NEWTON && (w->max_m > EXPLICIT_MAX_BODIES || w->max_j > EXPLICIT_MAX_JOINTS))
+        return fail(EXG_ERR_UNSUPPORTED, "exg_world_step: the explicit integrators take islands of up to 8 bodies "
+                                         "and 16 joints");
     w->ev_used = 0;
     w->graph_ms_valid = false;
     if (w->use_graph && n_steps >= 2) {

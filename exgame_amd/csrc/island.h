@@ -45,6 +45,26 @@ static_assert(sizeof(IslandTopo) == 320, "IslandTopo is a 320-byte record");
 int build_island_topology(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
                           IslandTopo& t);
 
+// The same tables for mechanism-sized islands (more than 8 bodies or 16 joints, up
+// to LARGE_MAXM bodies and LARGE_MAXJ joints), one record per such island. Neighbour
+// entries are indexed in block-row order (the non-zero blocks of A); assembly terms
+// are listed per neighbour entry.
+constexpr int LARGE_MAXM = 48, LARGE_MAXJ = 128, LARGE_MAXN = 6 * LARGE_MAXM, LARGE_MAXT = LARGE_MAXN / 16;
+constexpr int LARGE_MAXNBR = LARGE_MAXM + 2 * LARGE_MAXJ;
+struct LargeTopo {
+    uint16_t nbr_start[LARGE_MAXM + 1];
+    uint8_t nbr[LARGE_MAXNBR], nbr_row[LARGE_MAXNBR];     // column and row block of each non-zero block
+    uint16_t term_start[LARGE_MAXNBR + 1], terms[4 * LARGE_MAXJ];   // joint << 2 | row end << 1 | column end
+    uint16_t inc_start[LARGE_MAXM + 1], inc[2 * LARGE_MAXJ];       // joint << 1 | end
+    uint32_t kmask[LARGE_MAXT][3];                        // per 16-row tile: its non-zero 4-column slabs
+};
+// m bodies, J joints with island-local ends; returns 0, or -1 beyond the record.
+int build_large_topology(int m, int J, const int* ja, const int* jb, LargeTopo& t);
+// Doubles of device workspace one large island of m bodies and J joints uses: the
+// system matrices A, X, X', T (n16 × n16 each, n16 = 6m rounded up to 16) and the
+// joint Jacobians (J × 288).
+long long large_workspace_doubles(int m, int J);
+
 constexpr int ISLAND_THREADS = 256;
 // Per-island counters: spans, Newton iterations, NS iterations, reseeds, line-search
 // rejects, unconverged commits, steps that ended with a non-finite state (the fatal
@@ -82,6 +102,9 @@ struct IslandArgs {
     int jac_f32;              // 1: FP32 Jacobian lanes and compensated FP32 assembly (mixed precision)
     const IslandTopo* topo;   // I records (islands of the single-body class need none)
     int* fatal;               // island-steps that ended with a non-finite state (one counter); may be null
+    const LargeTopo* ltopo;   // per island, used by the large class only (others: unused entries)
+    double* lws;              // workspace of the large islands
+    const long long* lws_off; // per island: its offset into lws (large class only)
 };
 
 // Mark-and-continue fatal report of one island-step (the state is written as it is):
@@ -93,7 +116,10 @@ __device__ inline void mark_fatal(const IslandArgs& a, int isl) {
 
 // Island classes of the fused Newton kernel and the launch of one class over the
 // island ids in a.island_ids[0 .. a.n_islands).
-enum { ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASSES = 4 };
+enum {
+    ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASS_LARGE = 4,
+    ISLAND_CLASSES = 5
+};
 int island_class(int m, int joints);   // -1 when no variant holds the island
 // Waves per workgroup and MFMA tiles per wave of a class's kernel (for its tile lists).
 void island_class_shape(int cls, int& waves, int& per_wave);
@@ -113,6 +139,9 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t stream, hipEve
 // (hipExtLaunchKernelGGL: no marker packets between kernels); null records nothing.
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start = nullptr,
                          hipEvent_t stop = nullptr);
+// Mechanism-sized islands, one workgroup of LARGE_THREADS per island (island_large.hip).
+constexpr int LARGE_THREADS = 512;
+void launch_island_large(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Single-body islands without joints, one thread per island (singles.hip).
 void launch_singles(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 

@@ -1049,6 +1049,7 @@ int island_class(int m, int j) {
     if (m <= 4 && j <= 8) return ISLAND_CLASS_M4J8;
     if (m <= 8 && j <= 14) return ISLAND_CLASS_M8J14;
     if (m <= 8 && j <= 16) return ISLAND_CLASS_M8J16;
+    if (m <= LARGE_MAXM && j <= LARGE_MAXJ) return ISLAND_CLASS_LARGE;
     return -1;
 }
 
@@ -1085,6 +1086,9 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEv
         case ISLAND_CLASS_M8J14:
             if (a.jac_f32) launch_k(island_step_kernel_m8j14_jf32, grid, dim3(256), stream, start, stop, a);
             else launch_k(island_step_kernel_m8j14, grid, dim3(256), stream, start, stop, a);
+            break;
+        case ISLAND_CLASS_LARGE:
+            launch_island_large(a, stream, start, stop);
             break;
         default:
             if (a.jac_f32) launch_k(island_step_kernel_m8j16_jf32, grid, dim3(256), stream, start, stop, a);

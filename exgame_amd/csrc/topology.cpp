@@ -16,6 +16,7 @@
 #include "island.h"
 
 #include <cstring>
+#include <vector>
 
 namespace exg {
 
@@ -131,10 +132,76 @@ int build_island_topology(int m, int J, const int* ja, const int* jb, const int*
     return 0;
 }
 
+int build_large_topology(int m, int J, const int* ja, const int* jb, LargeTopo& t) {
+    std::memset(&t, 0, sizeof t);
+    if (m < 1 || m > LARGE_MAXM || J < 0 || J > LARGE_MAXJ) return -1;
+    for (int j = 0; j < J; ++j)
+        if (ja[j] < 0 || ja[j] >= m || jb[j] < 0 || jb[j] >= m || ja[j] == jb[j]) return -1;
+    // neighbour lists: body i first, then the other ends of its joints in joint order
+    std::vector<std::vector<int>> lists(m);
+    for (int i = 0; i < m; ++i) lists[i].push_back(i);
+    auto add = [&](int i, int k) {
+        for (int x : lists[i])
+            if (x == k) return;
+        lists[i].push_back(k);
+    };
+    for (int j = 0; j < J; ++j) {
+        add(ja[j], jb[j]);
+        add(jb[j], ja[j]);
+    }
+    int o = 0;
+    for (int i = 0; i < m; ++i) {
+        t.nbr_start[i] = (uint16_t)o;
+        for (int k : lists[i]) {
+            if (o >= LARGE_MAXNBR) return -1;
+            t.nbr[o] = (uint8_t)k;
+            t.nbr_row[o] = (uint8_t)i;
+            ++o;
+        }
+    }
+    t.nbr_start[m] = (uint16_t)o;
+    // assembly terms of each non-zero block (bi, bj), in joint order
+    int to = 0;
+    for (int q = 0; q < o; ++q) {
+        t.term_start[q] = (uint16_t)to;
+        const int bi = t.nbr_row[q], bj = t.nbr[q];
+        for (int j = 0; j < J; ++j) {
+            const int ends[2] = {ja[j], jb[j]};
+            for (int re = 0; re < 2; ++re)
+                for (int ce = 0; ce < 2; ++ce)
+                    if (ends[re] == bi && ends[ce] == bj) t.terms[to++] = (uint16_t)((j << 2) | (re << 1) | ce);
+        }
+    }
+    t.term_start[o] = (uint16_t)to;
+    // incidence
+    o = 0;
+    for (int i = 0; i < m; ++i) {
+        t.inc_start[i] = (uint16_t)o;
+        for (int j = 0; j < J; ++j) {
+            if (ja[j] == i) t.inc[o++] = (uint16_t)(j << 1);
+            if (jb[j] == i) t.inc[o++] = (uint16_t)((j << 1) | 1);
+        }
+    }
+    t.inc_start[m] = (uint16_t)o;
+    // slab masks: body bi's rows meet body bk's columns
+    for (int bi = 0; bi < m; ++bi)
+        for (int q = t.nbr_start[bi]; q < t.nbr_start[bi + 1]; ++q) {
+            const int bk = t.nbr[q];
+            for (int tt = (6 * bi) / 16; tt <= (6 * bi + 5) / 16; ++tt)
+                for (int ks = (6 * bk) / 4; ks <= (6 * bk + 5) / 4; ++ks) t.kmask[tt][ks >> 5] |= 1u << (ks & 31);
+        }
+    return 0;
+}
+
+long long large_workspace_doubles(int m, int J) {
+    const long long n16 = ((6LL * m + 15) / 16) * 16;
+    return 4 * n16 * n16 + 288LL * J;
+}
+
 int island_topology_of(int m, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t) {
     const int cls = island_class(m, J);
     if (cls < 0) return -1;
-    if (cls == ISLAND_CLASS_M1) {
+    if (cls == ISLAND_CLASS_M1 || cls == ISLAND_CLASS_LARGE) {   // no fixed record (large: LargeTopo)
         std::memset(&t, 0, sizeof t);
         return 0;
     }

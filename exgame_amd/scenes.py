@@ -249,6 +249,65 @@ def jointed_chains(n_chains: int = 512, m: int = 8, seed: int = 0, radius: float
                       "braced": braced})
 
 
+def cloth_grids(shapes=((4, 4), (5, 9)), seed: int = 0, radius: float = 20.0, k: float = 1e3, c: float = 10.0,
+                c_tors: float = 5.0, eps: float = 1e-6, chains=()) -> Scene:
+    """Mechanism-sized islands: one cloth per (rows, cols) in `shapes`, every body
+    dynamic. A cloth is a grid of rigid bodies at spacing 1 × U[0.95, 1.05] in a
+    random plane, jittered by U[−0.02, 0.02] per axis; axial spring-dampers join
+    grid neighbours along both axes (rest = the initial distance), torsional dampers
+    follow the rows, and the first cell carries a critically damped warped diagonal
+    and a perpendicular damper across the other diagonal. A 5 × 9 cloth has 45
+    bodies and 118 joints, a 4 × 4 cloth 16 and 38. `chains` adds chains of the
+    given lengths (the jointed-chain joints: 2(m − 1) per chain)."""
+    st = Stream(seed, 5)
+    positions, rots, masses, tensors, vbs, sizes, joints = [], [], [], [], [], [], []
+    body = 0
+    centers = st.in_ball(len(shapes), radius)
+    R = rotation_matrix(st.rotations(len(shapes)))
+    for q, (rows, cols) in enumerate(shapes):
+        m = rows * cols
+        gap = st.uniform(0.95, 1.05, 2)
+        grid = np.array([[gap[0] * (r - 0.5 * (rows - 1)), gap[1] * (cc - 0.5 * (cols - 1)), 0.0]
+                         for r in range(rows) for cc in range(cols)])
+        grid += st.uniform(-0.02, 0.02, 3 * m).reshape(m, 3)
+        p = centers[q][None] + grid @ R[q].T
+        mass, tens, rot, vb = rigid_draws(st, m)
+        positions.append(p); rots.append(rot); masses.append(mass); tensors.append(tens); vbs.append(vb)
+
+        def at(r, cc):
+            return body + r * cols + cc
+
+        def dist(u, v):
+            return float(np.linalg.norm(p[u - body] - p[v - body]))
+
+        for r in range(rows):
+            for cc in range(cols):
+                if cc + 1 < cols:
+                    u, v = at(r, cc), at(r, cc + 1)
+                    joints.append((SIMPLE_SPRING_DAMPER, u, v, (0, 0, 0, 0, 0, 0, dist(u, v), k, c, eps)))
+                    joints.append((TORSIONAL_DAMPER_WARPED, u, v, (c_tors,)))
+                if r + 1 < rows:
+                    u, v = at(r, cc), at(r + 1, cc)
+                    joints.append((SIMPLE_SPRING_DAMPER, u, v, (0, 0, 0, 0, 0, 0, dist(u, v), k, c, eps)))
+        if rows >= 2 and cols >= 2:
+            u, v = at(0, 0), at(1, 1)
+            joints.append((CRITICALLY_DAMPED_WARPED, u, v, (0, 0, 0, 0, 0, 0, dist(u, v), 0.5 * k, c, eps)))
+            joints.append((PERPENDICULAR_DAMPER_WARPED, at(0, 1), at(1, 0), (0.5 * c, eps)))
+        sizes.append(m)
+        body += m
+    for m in chains:
+        p, links = chain_bodies(st, 1, m, radius)
+        mass, tens, rot, vb = rigid_draws(st, m)
+        positions.append(p); rots.append(rot); masses.append(mass); tensors.append(tens); vbs.append(vb)
+        joints += chain_joints(body, links[0], k, c, c_tors, eps)
+        sizes.append(m)
+        body += m
+    return _assemble(np.concatenate(positions), np.concatenate(rots), np.concatenate(masses),
+                     np.concatenate(tensors), np.concatenate(vbs), np.array(sizes, dtype=np.int64), joints,
+                     {"workload": "cloth_grids", "shapes": [list(s) for s in shapes], "chains": list(chains),
+                      "seed": seed, "radius": radius})
+
+
 def mixed_world(n_bodies: int, seed: int, radius: float | None = None, rigs: bool = False, center=(0.0, 0.0, 0.0),
                 k: float = 1e3, c: float = 10.0, c_tors: float = 5.0, eps: float = 1e-6) -> Scene:
     """60% free gravitating bodies (singleton islands) and 40% jointed 8-body chains;

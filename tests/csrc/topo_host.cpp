@@ -17,3 +17,24 @@ extern "C" int tp_build(int m, int J, const int* ja, const int* jb, const int* k
     return rc;
 }
 extern "C" int tp_size() { return (int)sizeof(exg::IslandTopo); }
+
+// the mechanism-sized record: tp_build_large fills one LargeTopo; tp_large_layout
+// writes the byte offsets of its eight fields and its size
+extern "C" int tp_build_large(int m, int J, const int* ja, const int* jb, unsigned char* out) {
+    exg::LargeTopo t;
+    const int rc = exg::build_large_topology(m, J, ja, jb, t);
+    std::memcpy(out, &t, sizeof t);
+    return rc;
+}
+extern "C" void tp_large_layout(long long* out) {
+    out[0] = offsetof(exg::LargeTopo, nbr_start);
+    out[1] = offsetof(exg::LargeTopo, nbr);
+    out[2] = offsetof(exg::LargeTopo, nbr_row);
+    out[3] = offsetof(exg::LargeTopo, term_start);
+    out[4] = offsetof(exg::LargeTopo, terms);
+    out[5] = offsetof(exg::LargeTopo, inc_start);
+    out[6] = offsetof(exg::LargeTopo, inc);
+    out[7] = offsetof(exg::LargeTopo, kmask);
+    out[8] = sizeof(exg::LargeTopo);
+}
+extern "C" long long tp_large_workspace(int m, int J) { return exg::large_workspace_doubles(m, J); }

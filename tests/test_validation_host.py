@@ -44,3 +44,24 @@ def test_create_refuses_a_rotorless_pose():
     with pytest.raises(_lib.ExgError) as e:
         DeviceWorld(s)
     assert e.value.code == 6 and "rotor" in str(e.value)
+
+
+def test_create_refuses_islands_beyond_the_large_record():
+    # 49 bodies in one island (one more than the mechanism-sized kernel holds)
+    s = scenes.cloth_grids(shapes=((7, 7),))
+    with pytest.raises(_lib.ExgError) as e:
+        DeviceWorld(s)
+    assert e.value.code == 7 and "48" in str(e.value)
+    # 129 joints between 10 bodies
+    s = scenes.jointed_chains(n_chains=1, m=10, seed=1)
+    extra = 129 - s.n_joints
+    kind = np.concatenate([s.joint_kind, np.full(extra, scenes.TORSIONAL_DAMPER_WARPED, np.int32)])
+    ja = np.concatenate([s.joint_a, np.zeros(extra, np.int32)])
+    jb = np.concatenate([s.joint_b, np.ones(extra, np.int32)])
+    prm = np.concatenate([s.joint_params, np.tile([[5.0] + [0.0] * 9], (extra, 1))])
+    for k, v in (("joint_kind", kind), ("joint_a", ja), ("joint_b", jb), ("joint_params", prm),
+                 ("island_joint_start", np.array([0, 129], np.int32))):
+        object.__setattr__(s, k, v)
+    with pytest.raises(_lib.ExgError) as e:
+        DeviceWorld(s)
+    assert e.value.code == 7 and "128" in str(e.value)
