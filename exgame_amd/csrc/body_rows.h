@@ -72,10 +72,11 @@ __device__ __forceinline__ double pick6(double v0, double v1, double v2, double 
     return out;
 }
 
-__device__ __forceinline__ void body_post_row(const double pose_mid[8], const double vel[6], double mass,
-                                              const double t9[9], const double snap[6], const double total[6],
-                                              double half, double floor2, int row, double brow[6], double rhs[6],
-                                              double scale2[2]) {
+// The first part of body_post_row, which does not depend on the applied wrench: row
+// `row` of the world mass block and the momentum P = block·V (every lane of the group
+// ends with all six entries).
+__device__ __forceinline__ void body_post_row_momentum(const double pose_mid[8], const double vel[6], double mass,
+                                                       const double t9[9], int row, double brow[6], double p[6]) {
     double inv[8], adi[6][6], cad[6][6];
     motor_rev(pose_mid, inv);
     ad_matrix(inv, adi);
@@ -99,10 +100,15 @@ __device__ __forceinline__ void body_post_row(const double pose_mid[8], const do
     double pr = 0.0;
 #pragma unroll
     for (int c = 0; c < 6; ++c) pr = pr + brow[c] * vel[c];
-    double p[6], hh[6];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) p[r] = group8(pr, r);
+}
+// The rest: residual −(P − snap − half·total) and the self-scale.
+__device__ __forceinline__ void body_post_row_residual(const double p[6], const double snap[6], const double total[6],
+                                                       double half, double floor2, double rhs[6], double scale2[2]) {
+    double hh[6];
 #pragma unroll
     for (int r = 0; r < 6; ++r) {
-        p[r] = group8(pr, r);
         hh[r] = total[r] * half;
         rhs[r] = -(p[r] - snap[r] - hh[r]);
     }
@@ -114,6 +120,15 @@ __device__ __forceinline__ void body_post_row(const double pose_mid[8], const do
         const double sh = hh[b] * hh[b] + hh[b + 1] * hh[b + 1] + hh[b + 2] * hh[b + 2];
         scale2[o] = fmax(fmax(fmax(sp_, sn), sh), floor2);
     }
+}
+
+__device__ __forceinline__ void body_post_row(const double pose_mid[8], const double vel[6], double mass,
+                                              const double t9[9], const double snap[6], const double total[6],
+                                              double half, double floor2, int row, double brow[6], double rhs[6],
+                                              double scale2[2]) {
+    double p[6];
+    body_post_row_momentum(pose_mid, vel, mass, t9, row, brow, p);
+    body_post_row_residual(p, snap, total, half, floor2, rhs, scale2);
 }
 
 }  // namespace exg

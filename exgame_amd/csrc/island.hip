@@ -130,6 +130,12 @@ struct IslandSolver {
     IslandShared<MAXM, MAXJ, NTH, JF>& s;
     const IslandArgs& a;
     static constexpr int LD = IslandShared<MAXM, MAXJ, NTH, JF>::LDM;
+    // Line-search rounds with the joints on a wave of their own: with eight bodies per
+    // candidate (one wave) and four or more waves, three candidates take a wave each
+    // and the fourth wave evaluates their joints (3·MAXJ ≤ 64 items); otherwise LS
+    // candidates per round in three phases (pre_candidates, joints_plain, gather_post).
+    static constexpr bool SPLIT = NTH / 64 >= 4 && 8 * MAXM == 64 && 3 * MAXJ <= 64;
+    static constexpr int LSR = SPLIT ? 3 : LS;
     int m, J, isl, first_body;
     int n, n16;               // system size 6m and its 16-padded size
     bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
@@ -302,6 +308,121 @@ struct IslandSolver {
             }
             if ((t % LSTR) == 0 && ll < nl) s.wn[l0 + ll] = worst;
         }
+    }
+    // ───── line-search round, joints beside the body rows (SPLIT classes) ─────
+    // Phase 1: wave w < nl takes candidate w = round·LSR + w (vmid = base + 2^-w·dv),
+    // eight threads per body; each forms the candidate's velocity and midpoint (the
+    // same values on all eight; lane 0 of the group stores them), one row of the world
+    // mass block and the momentum P = block·V, none of which depend on the joints. The
+    // waves from nl on evaluate every (candidate, joint) wrench, forming the two ends'
+    // candidate velocities and midpoints again from the same inputs by the same code
+    // (so with the same bits). P waits in the candidate's rhs slot for phase 2 (no
+    // registers held across the barrier); candidate 0's mass block goes to mass0 when
+    // mass0 may take it (as in gather_post).
+    __device__ void ls_round_bodies_joints(int round, int nl, bool mass_of_candidate0) {
+        const int t = tid(), w = uni(t >> 6);
+        if (w < nl) {
+            const int i = (t & 63) >> 3, r = t & 7;
+            if (i < m) {
+                const double alpha = ldexp(1.0, -(round * LSR + w));
+                double v[6], e[8], mid[8];
+#pragma unroll
+                for (int k = 0; k < 6; ++k) v[k] = s.base[i][k] + s.dv[i][k] * alpha;
+                twist_exp(v, half, e);
+                compose(s.pose[i], e, mid);
+                if (r == 0) {
+                    double* vm = VMID(w, i);
+                    double* md = MID(w, i);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) vm[k] = v[k];
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) md[k] = mid[k];
+                }
+                double brow[6], p[6];
+                body_post_row_momentum(mid, v, s.massv[i], s.tens[i], r < 6 ? r : 5, brow, p);
+                if (r == 0) {
+                    double* rh = RHS(w, i);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) rh[k] = p[k];
+                }
+                if (w == 0 && mass_of_candidate0 && r < 6)
+                    for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = brow[c];
+            }
+        } else if constexpr (MAXJ > 0) {
+            const int q = t - 64 * nl;
+            if (q < nl * J) {
+                const int lane = q % nl, j = s.jorder[q / nl];
+                const double alpha = ldexp(1.0, -(round * LSR + lane));
+                const int ends[2] = {s.ja[j], s.jb[j]};
+                double v[2][6], mid[2][8];
+#pragma unroll
+                for (int en = 0; en < 2; ++en) {
+                    const int b = ends[en];
+                    double e[8];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) v[en][k] = s.base[b][k] + s.dv[b][k] * alpha;
+                    twist_exp(v[en], half, e);
+                    compose(s.pose[b], e, mid[en]);
+                }
+                joint_wrench(s.jkind[j], s.jparams[j], mid[0], mid[1], v[0], v[1], dt, warp, CONN(lane, j, 0),
+                             CONN(lane, j, 1));
+            }
+        }
+    }
+    // Phase 2 (after a barrier): gather each candidate body's wrench (external +
+    // incident in joint order), finish its residual and self-scale from P, and fold the
+    // candidate's worst residual ratio across its wave into s.wn[w].
+    __device__ void ls_round_gather(int nl) {
+        const int t = tid(), w = uni(t >> 6);
+        if (w >= nl) return;
+        const int i = (t & 63) >> 3, r = t & 7;
+        double r0 = 0.0, r1 = 0.0;
+        if (i < m) {
+            double tot[6];
+            for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
+            for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
+                const int code = s.inc[q];
+                const double* wr = CONN(w, code >> 1, code & 1);
+                for (int k = 0; k < 6; ++k) tot[k] = tot[k] + wr[k];
+            }
+            const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
+            double rhs[6], sc[2], p[6];
+            const double* pp = RHS(w, i);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) p[k] = pp[k];
+            body_post_row_residual(p, s.snap[i], tot, half, floor2, rhs, sc);
+            if (r == 0) {
+                double* tl = TOTAL(w, i);
+                double* rh = RHS(w, i);
+                double* scl = SCALE(w, i);
+                for (int k = 0; k < 6; ++k) {
+                    tl[k] = tot[k];
+                    rh[k] = rhs[k];
+                }
+                scl[0] = sc[0];
+                scl[1] = sc[1];
+            }
+            const double lin2 = rhs[0] * rhs[0] + rhs[1] * rhs[1] + rhs[2] * rhs[2];
+            const double ang2 = rhs[3] * rhs[3] + rhs[4] * rhs[4] + rhs[5] * rhs[5];
+            r0 = lin2 / sc[0];
+            r1 = ang2 / sc[1];
+        }
+        double worst = 0.0;
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q)
+            if (q < m) {
+                const double a0 = __longlong_as_double(
+                    (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
+                                     (int)(__double_as_longlong(r0) >> 32), q * 8) << 32) |
+                                (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(r0), q * 8)));
+                const double a1 = __longlong_as_double(
+                    (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
+                                     (int)(__double_as_longlong(r1) >> 32), q * 8) << 32) |
+                                (unsigned)__builtin_amdgcn_readlane((int)__double_as_longlong(r1), q * 8)));
+                worst = max2(worst, a0);
+                worst = max2(worst, a1);
+            }
+        if ((t & 63) == 0) s.wn[w] = worst;
     }
     // ───── system matrix: A = 0·A + [i==j]·𝕀 + Σ terms (pf·∂W/∂pose + vf·∂W/∂vel) ─────
     __device__ void assemble() {
@@ -705,16 +826,24 @@ struct IslandSolver {
             // the matvec and published by its barrier)
             int accepted = -1, accepted_slot = -1;
             double tnorm = 0.0;
-            for (int round = 0; round < LANES / LS && accepted < 0; ++round) {
-                pre_candidates(round);
-                __syncthreads();
-                joints_plain(0, LS);
-                __syncthreads();
-                gather_post(0, LS, round == 0);
-                __syncthreads();
-                for (int l = 0; l < LS; ++l)
+            for (int round = 0; round * LSR < LANES && accepted < 0; ++round) {
+                const int nl = LANES - round * LSR < LSR ? LANES - round * LSR : LSR;
+                if constexpr (SPLIT) {
+                    ls_round_bodies_joints(round, nl, round == 0);
+                    __syncthreads();
+                    ls_round_gather(nl);
+                    __syncthreads();
+                } else {
+                    pre_candidates(round);
+                    __syncthreads();
+                    joints_plain(0, LS);
+                    __syncthreads();
+                    gather_post(0, LS, round == 0);
+                    __syncthreads();
+                }
+                for (int l = 0; l < nl; ++l)
                     if (uni(s.wn[l]) < rnorm) {
-                        accepted = round * LS + l;
+                        accepted = round * LSR + l;
                         accepted_slot = l;
                         tnorm = uni(s.wn[l]);
                         break;
