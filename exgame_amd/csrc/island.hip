@@ -130,6 +130,7 @@ struct IslandSolver {
     IslandShared<MAXM, MAXJ, NTH, JF>& s;
     const IslandArgs& a;
     static constexpr int LD = IslandShared<MAXM, MAXJ, NTH, JF>::LDM;
+    using JT = typename IslandShared<MAXM, MAXJ, NTH, JF>::JT;
     // Line-search rounds with the joints on a wave of their own: with eight bodies per
     // candidate (one wave) and four or more waves, three candidates take a wave each
     // and the fourth wave evaluates their joints (3·MAXJ ≤ 64 items); otherwise LS
@@ -190,20 +191,42 @@ struct IslandSolver {
     // ───── Jacobian lanes: one thread per (joint, axis) ─────
     // item q < 12J: pose column q % 12 of joint q / 12 (in kind order); item 12J + q:
     // velocity column 12 + q % 12 of the same joint
-    __device__ void jacobian_item(int item) {
+    // With `mirrored`, velocity item q < 6J is column 12 + q % 6 of joint q / 6 (end a's
+    // velocity axes), stored also as end b's column 18 + q % 6 negated.
+    __device__ void jacobian_item(int item, bool mirrored = false) {
         const int pose_item = item < J * 12;
         const int q = pose_item ? item : item - J * 12;
-        const int j = s.jorder[q / 12], ax = (pose_item ? 0 : 12) + q % 12, ea = s.ja[j], eb = s.jb[j];
+        const int per = (pose_item || !mirrored) ? 12 : 6;
+        const int j = s.jorder[q / per], ax = (pose_item ? 0 : 12) + q % per, ea = s.ja[j], eb = s.jb[j];
         // the f64 lanes also deliver the wrench values; with f32 lanes the residual's
         // wrenches come from an f64 value-only evaluation instead
         const bool values = !JF && ax == 0;
-        joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp,
-                     s.u.jac[j][0][ax], s.u.jac[j][1][ax], values ? s.conn0[j][0] : nullptr,
-                     values ? s.conn0[j][1] : nullptr);
+        JT da[6], db[6];
+        joint_column(s.jkind[j], s.jparams[j], s.mid0[ea], s.mid0[eb], s.vmid0[ea], s.vmid0[eb], ax, dt, warp, da, db,
+                     values ? s.conn0[j][0] : nullptr, values ? s.conn0[j][1] : nullptr);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            s.u.jac[j][0][ax][c] = da[c];
+            s.u.jac[j][1][ax][c] = db[c];
+        }
+        if (!pose_item && mirrored) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                s.u.jac[j][0][ax + 6][c] = -da[c];
+                s.u.jac[j][1][ax + 6][c] = -db[c];
+            }
+        }
     }
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
         const int P = J * 12, t = tid();
+        if (P + 6 * J <= NTH) {
+            // one column per thread: the pose columns, then the velocity columns of
+            // end a, each also giving end b's column by negation (vrel = va − vb enters
+            // the wrenches linearly, so ∂W/∂vb = −∂W/∂va exactly)
+            if (t < P + 6 * J) jacobian_item(t, true);
+            return;
+        }
         int first, second;
         if (3 * P <= 2 * NTH) {
             // one pose column (tangents through the geometry) per thread; the cheaper
@@ -787,20 +810,25 @@ struct IslandSolver {
         for (int iter = 0; iter < MAX_ITERATIONS; ++iter) {
             double rnorm;
             st_newton++;
-            if (has_carried) {
-                has_carried = false;
-                joints_jacobian();
-                __syncthreads();
-                rnorm = carried;
-            } else {
+            // the Jacobian at the iterate: after an accepted line-search lane the
+            // iterate's midpoints, wrenches and residual are that lane's; otherwise
+            // they are formed here (one call site of the Jacobian code)
+            const bool fresh = !has_carried;
+            has_carried = false;
+            if (fresh) {
                 pre(0, 1);
                 __syncthreads();
-                joints_jacobian();
-                if constexpr (JF) joints_plain(0, 1);
-                __syncthreads();
+            }
+            joints_jacobian();
+            if constexpr (JF)
+                if (fresh) joints_plain(0, 1);
+            __syncthreads();
+            if (fresh) {
                 gather_post(0, 1);
                 __syncthreads();
                 rnorm = uni(s.wn[0]);
+            } else {
+                rnorm = carried;
             }
             if (rnorm < best) {
                 best = rnorm;
