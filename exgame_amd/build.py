@@ -14,8 +14,10 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "libexgame_hip.so")
 
-SOURCES = ["island.hip", "island_large.hip", "singles.hip", "explicit.hip", "gravity.hip", "rows.hip", "exg_api.cpp", "store.cpp", "topology.cpp",
-           "arena.cpp", "noalloc.hip"]
+SOURCES = ["island.hip", "island_large.hip", "singles.hip", "explicit.hip", "gravity.hip", "rows.hip", "rows_traced.hip",
+           "exg_api.cpp", "store.cpp", "topology.cpp", "arena.cpp", "noalloc.hip"]
+VIETE = os.path.join(HERE, "viete")
+GENERATED = os.path.join(CSRC, "viete_gen.h")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # -packed-fp32-ops: no packed FP32 instructions anywhere in this library.
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Xclang", "-target-feature", "-Xclang",
@@ -31,8 +33,27 @@ def _compile(src: str) -> str:
     return obj
 
 
+def generate() -> str:
+    """Build the tracer (host code) and write csrc/viete_gen.h: the joint laws and the
+    Pre stage of csrc/physics_dev.h traced and emitted as device functions."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    exe = os.path.join(OUT_DIR, "viete_gen")
+    cmd = [HIPCC, "-x", "hip", "--offload-host-only", "-O1", "-std=c++17", "-ffp-contract=off", "-I", CSRC,
+           os.path.join(VIETE, "gen.cpp"), "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"the tracer did not build:\n{r.stderr[-6000:]}")
+    tmp = GENERATED + ".tmp"
+    r = subprocess.run([exe, tmp], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"the tracer failed:\n{r.stderr[-6000:]}")
+    os.replace(tmp, GENERATED)
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
+    generate()
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"

@@ -180,10 +180,10 @@ __host__ __device__ inline Dual<float> promote<Dual<float>, float>(const float& 
 // weight and coordinates): line, soft length, unit wrench, length rate, force law
 // (axial_spring_damper/mod.rs:315-385; the laws at :47-61 and
 // critically_damped_warped.rs:58-90).
-template <class SP, class SV>
-__host__ __device__ inline void axial_from_points(int kind, const double* params, const SP& wa, const SP qa[3],
-                                                  const SP& wb, const SP qb[3], const SV vrel[6], double dt,
-                                                  double warp, SV wa_out[6], SV wb_out[6]) {
+template <class SP, class SV, class P>
+__host__ __device__ inline void axial_from_points(int kind, const P* params, const SP& wa, const SP qa[3],
+                                                  const SP& wb, const SP qb[3], const SV vrel[6], const P& dt,
+                                                  const P& warp, SV wa_out[6], SV wb_out[6]) {
     using RP = typename RealOf<SP>::type;
     using RV = typename RealOf<SV>::type;
     SP l[6];
@@ -219,9 +219,9 @@ __host__ __device__ inline void axial_from_points(int kind, const double* params
 // The perpendicular damper's wrenches from its two world anchor points (the COMs):
 // soft length, unit direction, the perpendicular part of the relative velocity at a
 // (perpendicular_damper_warped.rs:107-162).
-template <class SP, class SV>
-__host__ __device__ inline void perp_from_points(const double* params, const SP& wap, const SP qap[3], const SP& wb,
-                                                 const SP qb[3], const SV vrel[6], double warp, SV wa_out[6],
+template <class SP, class SV, class P>
+__host__ __device__ inline void perp_from_points(const P* params, const SP& wap, const SP qap[3], const SP& wb,
+                                                 const SP qb[3], const SV vrel[6], const P& warp, SV wa_out[6],
                                                  SV wb_out[6]) {
     using RP = typename RealOf<SP>::type;
     using RV = typename RealOf<SV>::type;
@@ -258,10 +258,12 @@ __host__ __device__ inline void perp_from_points(const double* params, const SP&
 // The geometry (poses) is computed in SP and the velocity-dependent part in SV; with
 // SP = SV this is one type throughout. A Jacobian column along a velocity axis needs
 // tangents only in SV, so its geometry runs on plain values.
-template <class SP, class SV>
-__host__ __device__ inline void joint_wrench2(int kind, const double* params, const SP pa_pose[8],
-                                              const SP pb_pose[8], const SV va[6], const SV vb[6], double dt,
-                                              double warp, SV wa_out[6], SV wb_out[6]) {
+// `params`, `dt` and `warp` are doubles in the kernels (P = double) and symbolic
+// inputs when the law is traced for code generation.
+template <class SP, class SV, class P>
+__host__ __device__ inline void joint_wrench2(int kind, const P* params, const SP pa_pose[8],
+                                              const SP pb_pose[8], const SV va[6], const SV vb[6], const P& dt,
+                                              const P& warp, SV wa_out[6], SV wb_out[6]) {
     using RP = typename RealOf<SP>::type;
     using RV = typename RealOf<SV>::type;
     SV vrel[6];
@@ -291,11 +293,11 @@ __host__ __device__ inline void joint_wrench2(int kind, const double* params, co
     }
 }
 
-template <class S>
-__host__ __device__ inline void joint_wrench(int kind, const double* params, const S pa_pose[8], const S pb_pose[8],
-                                             const S va[6], const S vb[6], double dt, double warp, S wa_out[6],
+template <class S, class P>
+__host__ __device__ inline void joint_wrench(int kind, const P* params, const S pa_pose[8], const S pb_pose[8],
+                                             const S va[6], const S vb[6], const P& dt, const P& warp, S wa_out[6],
                                              S wb_out[6]) {
-    joint_wrench2<S, S>(kind, params, pa_pose, pb_pose, va, vb, dt, warp, wa_out, wb_out);
+    joint_wrench2<S, S, P>(kind, params, pa_pose, pb_pose, va, vb, dt, warp, wa_out, wb_out);
 }
 
 // Derivative seeds of axis `axis` (0..23) for the unit retraction base∘exp(δ):

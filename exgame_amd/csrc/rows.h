@@ -47,5 +47,8 @@ struct StorePtrs {
 };
 
 void launch_rows(int kind, const StorePtrs& s, const uint32_t* row_slots, uint32_t count, hipStream_t stream);
+// The joint and Pre kinds on the traced kernels (rows_traced.hip); false for the
+// other kinds, which have no traced form.
+bool launch_rows_traced(int kind, const StorePtrs& s, const uint32_t* row_slots, uint32_t count, hipStream_t stream);
 
 }  // namespace exg

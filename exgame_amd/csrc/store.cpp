@@ -51,6 +51,7 @@ struct TypedMap {
 struct exg_store {
     TypedMap maps[exg::N_STORE_TYPES];
     int scalar_bytes = 8;                 // 4: f32 storages, 8: f64
+    int codegen = EXG_CODEGEN_HANDWRITTEN;  // joint and Pre rows: hand-written or traced kernels
     hipStream_t stream = nullptr;
     uint32_t* dev_slots = nullptr;
     size_t dev_slots_cap = 0;
@@ -216,6 +217,14 @@ int exg_store_create_scalar(uint64_t capacity, int scalar, exg_store** out) {
 }
 
 int exg_store_create(uint64_t capacity, exg_store** out) { return exg_store_create_scalar(capacity, EXG_SCALAR_F64, out); }
+
+int exg_store_set_codegen(exg_store* s, int codegen) {
+    if (!s) return sfail(EXG_ERR_INVALID, "exg_store_set_codegen: null store");
+    if (codegen != EXG_CODEGEN_HANDWRITTEN && codegen != EXG_CODEGEN_TRACED)
+        return sfail(EXG_ERR_INVALID, "exg_store_set_codegen: unknown mode " + std::to_string(codegen));
+    s->codegen = codegen;
+    return EXG_OK;
+}
 
 int exg_store_scalar(exg_store* s, int* scalar) {
     if (!s || !scalar) return sfail(EXG_ERR_INVALID, "exg_store_scalar: null argument");
@@ -384,7 +393,8 @@ int exg_dispatch(exg_store* s, int kind, const uint32_t* row_slots, uint32_t cou
     p.row = static_cast<const uint32_t*>(s->maps[exg::T_ROW].dev);
     p.fatal = s->dev_fatal;
     p.scalar_bytes = s->scalar_bytes;
-    exg::launch_rows(kind, p, s->dev_slots, count, s->stream);
+    if (!(s->codegen == EXG_CODEGEN_TRACED && exg::launch_rows_traced(kind, p, s->dev_slots, count, s->stream)))
+        exg::launch_rows(kind, p, s->dev_slots, count, s->stream);
     e = hipGetLastError();
     if (e != hipSuccess) return sfail(EXG_ERR_BACKEND, std::string("exg_dispatch: ") + hipGetErrorString(e));
     // the slot list is staged in one buffer: wait so the next dispatch can reuse it

@@ -35,12 +35,15 @@ JOINT_SHADER = {0: "SimpleSpringDamper", 1: "CriticallyDampedWarped", 2: "Perpen
 
 # storage scalar (World::builder().usual::<T>())
 SCALARS = {"f32": 0, "f64": 1}
+# kernels of the joint and Pre rows
+CODEGEN = {"handwritten": 0, "traced": 1}
 
 _u32p = C.POINTER(C.c_uint32)
 _SIGS = {
     "exg_store_create": [C.c_uint64, C.POINTER(C.c_void_p)],
     "exg_store_create_scalar": [C.c_uint64, C.c_int, C.POINTER(C.c_void_p)],
     "exg_store_scalar": [C.c_void_p, C.POINTER(C.c_int)],
+    "exg_store_set_codegen": [C.c_void_p, C.c_int],
     "exg_store_destroy": [C.c_void_p],
     "exg_slot_insert": [C.c_void_p, C.c_int, C.c_void_p, _u32p, _u32p],
     "exg_slot_read": [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_void_p, C.POINTER(C.c_int)],
@@ -131,6 +134,11 @@ class Store:
     def dispatch(self, kind: int, rows) -> None:
         slots = np.ascontiguousarray([r.index for r in rows], dtype=np.uint32)
         _lib.check(self.L.exg_dispatch(self.h, int(kind), slots.ctypes.data_as(_u32p), len(slots)))
+
+    def set_codegen(self, mode: str) -> None:
+        """Joint and Pre rows on the hand-written kernels or on the device functions
+        traced from the same laws at build time (exgame_amd/viete)."""
+        _lib.check(self.L.exg_store_set_codegen(self.h, CODEGEN[mode]))
 
     def fatals(self, kind: int) -> int:
         """Rows of the last dispatch of `kind` with non-finite outputs."""

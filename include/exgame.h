@@ -225,6 +225,14 @@ int exg_storage_device_ptr(exg_store* s, int type, void** ptr);
  * counted and the call returns EXG_ERR_FATAL after writing every output
  * (mark-and-continue, shaders.rs:411-431). */
 int exg_dispatch(exg_store* s, int kind, const uint32_t* row_slots, uint32_t count);
+/* Kernels of the joint (EXG_K_*_PLAIN / _JACOBIAN) and Pre rows: EXG_CODEGEN_HANDWRITTEN
+ * (default), or EXG_CODEGEN_TRACED — device functions generated at build time by
+ * tracing the same scalar-generic joint laws over a symbolic scalar (the reference's
+ * build.rs + viete route to its connection kernels, crates/newton/build.rs:673-821,
+ * crates/viete/src/glsl.rs:12-80), one thread per row. Same rows, slots and results
+ * to rounding. */
+enum { EXG_CODEGEN_HANDWRITTEN = 0, EXG_CODEGEN_TRACED = 1 };
+int exg_store_set_codegen(exg_store* s, int codegen);
 /* Wait for queued work (the fence wait of a flush). */
 int exg_flush(exg_store* s);
 /* Rows of the last dispatch of `kind` that produced non-finite outputs. */
