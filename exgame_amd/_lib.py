@@ -43,6 +43,7 @@ SIGNATURES = {
     "exg_world_destroy": [C.c_void_p],
     "exg_world_set_solver": [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_int],
     "exg_world_set_ns_engine": [C.c_void_p, C.c_int],
+    "exg_world_set_joint_mapping": [C.c_void_p, C.c_int],
     "exg_world_set_graph": [C.c_void_p, C.c_int],
     "exg_world_step": [C.c_void_p, C.c_int, C.c_double, C.c_double],
     "exg_world_sync": [C.c_void_p],

@@ -23,12 +23,17 @@ _SIGS = {
     "exg_arena_set_gravity_chunk": [C.c_void_p, C.c_int],
     "exg_arena_set_graph": [C.c_void_p, C.c_int],
     "exg_arena_autotune_gravity": [C.c_void_p, C.POINTER(C.c_int)],
+    "exg_arena_set_joint_mapping": [C.c_void_p, C.c_int],
+    "exg_arena_autotune_joints": [C.c_void_p, C.POINTER(C.c_int)],
     "exg_arena_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "exg_arena_admit": [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, _dp, C.c_int, _ip, _dp, C.c_int, _ip, _ip, _ip, _dp,
                         C.c_double, C.c_int, _ip],
     "exg_arena_retire": [C.c_void_p, C.c_int],
     "exg_arena_step": [C.c_void_p, C.c_int, C.c_double, C.c_double],
     "exg_arena_sync": [C.c_void_p],
+    "exg_arena_mark": [C.c_void_p, C.POINTER(C.c_uint64)],
+    "exg_arena_mark_done": [C.c_void_p, C.c_uint64, C.POINTER(C.c_int)],
+    "exg_arena_mark_wait": [C.c_void_p, C.c_uint64],
     "exg_arena_download": [C.c_void_p, C.c_int, _dp, _dp, _dp],
     "exg_arena_world_stats": [C.c_void_p, C.c_int, C.POINTER(C.c_int32)],
     "exg_arena_last_ms": [C.c_void_p, _dp],
@@ -94,6 +99,20 @@ class Arena:
         self.gravity_chunk = out.value
         return out.value
 
+    def set_joint_mapping(self, mapping: str) -> None:
+        """Joint mapping of the 8-body islands of worlds admitted from now on."""
+        from .world import JOINT_MAPPINGS
+        _lib.check(self.L.exg_arena_set_joint_mapping(self.h, JOINT_MAPPINGS[mapping]))
+
+    def autotune_joints(self) -> str:
+        """Time one step of the live worlds with all of them on each joint mapping (from a
+        saved state that is put back; the live worlds keep their mappings) and make the
+        fastest the mapping of later admissions; returns its name."""
+        from .world import JOINT_MAPPINGS
+        out = C.c_int(0)
+        _lib.check(self.L.exg_arena_autotune_joints(self.h, C.byref(out)))
+        return {v: k for k, v in JOINT_MAPPINGS.items()}[out.value]
+
     def admit(self, scene, G: float = 1.0, gravity: bool = True):
         """World id, or None when the arena has no room for it now."""
         s = scene
@@ -132,6 +151,20 @@ class Arena:
 
     def sync(self) -> None:
         _lib.check(self.L.exg_arena_sync(self.h))
+
+    def mark(self) -> int:
+        """A completion mark behind the work enqueued so far; returns its ticket."""
+        t = C.c_uint64()
+        _lib.check(self.L.exg_arena_mark(self.h, C.byref(t)))
+        return t.value
+
+    def mark_done(self, ticket: int) -> bool:
+        d = C.c_int()
+        _lib.check(self.L.exg_arena_mark_done(self.h, int(ticket), C.byref(d)))
+        return bool(d.value)
+
+    def mark_wait(self, ticket: int) -> None:
+        _lib.check(self.L.exg_arena_mark_wait(self.h, int(ticket)))
 
     def last_ms(self) -> float:
         v = C.c_double()

@@ -10,8 +10,10 @@ rollout, the device time per step, and the precision used.
                                  [--jacobian f64|f32c] [--chunk N | --autotune] [--graph]
 
 `--autotune` times the gravity pass under the chunk settings (0 = the single-world
-rule, 128 … 8192) on the admitted worlds, then admits the worlds afresh with the
-fastest; `--graph` replays two steps at a time from a HIP graph.
+rule, 128 … 8192) and one step under each joint mapping of the 8-body islands (from
+a saved state that is put back) on the admitted worlds, then admits the worlds
+afresh with the fastest of each; `--graph` replays two steps at a time from a HIP
+graph.
 """
 from __future__ import annotations
 
@@ -55,11 +57,14 @@ def main() -> int:
     if args.autotune:
         ar.step(1, 1e-2, 1.0)     # charges past the bootstrap frame
         ar.sync()
-        ar.autotune_gravity()     # applies to admissions from now on: admit the worlds again
+        ar.autotune_gravity()     # both apply to admissions from now on: admit the worlds again
+        joint_mapping = ar.autotune_joints()
         for i in ids:
             ar.retire(i)
         ids = [ar.admit(w) for w in worlds]
         assert all(i is not None for i in ids)
+    else:
+        joint_mapping = "split"
     ar.step(args.warmup, 1e-2, 1.0)
     ar.sync()
     t0 = time.perf_counter()
@@ -70,7 +75,8 @@ def main() -> int:
             "body_steps_per_s": B * args.steps / el, "ms_per_step": el * 1e3 / args.steps,
             "device_ms_per_step": ar.last_ms() / args.steps, "precision": args.precision,
             "jacobian": args.jacobian or args.precision,
-            "gravity_chunk": ar.gravity_chunk, "autotuned": bool(args.autotune), "graph": bool(args.graph),
+            "gravity_chunk": ar.gravity_chunk, "joint_mapping": joint_mapping, "autotuned": bool(args.autotune),
+            "graph": bool(args.graph),
             "islands": sum(w.n_islands for w in worlds), "pairs_per_step": int(sum(w.n_bodies ** 2 for w in worlds))}
     print(json.dumps(line), flush=True)
     return 0

@@ -86,6 +86,7 @@ struct ArenaWorld {
     double G = 1.0;
     int gravity_on = 1;
     int chunk = 0;   // sources per gravity chunk, fixed at admission (it sets the order of the force sums)
+    int mapping = EXG_JOINT_MAPPING_SPLIT;   // joint mapping of its 8-body islands, fixed at admission
 };
 
 }  // namespace
@@ -97,7 +98,9 @@ struct exg_arena {
     int grav_chunk = 0;   // chunk length of worlds admitted from now on; 0: the single-world rule
     double tol2 = 1e-8;
     int ns_warm = 2, ns_seed = 24, ns_engine = EXG_NS_MFMA, grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
+    int joint_mapping = EXG_JOINT_MAPPING_SPLIT;
     hipStream_t stream = nullptr;
+    exg::ClassStreams classes;     // the island classes run side by side
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G, grav_stage;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;                // island-steps that ended with a non-finite state
@@ -115,6 +118,9 @@ struct exg_arena {
     bool dirty = true;
     int front = 0;
     hipEvent_t ev[2] = {nullptr, nullptr};
+    static constexpr int N_MARKS = 8;
+    hipEvent_t marks[N_MARKS] = {};   // completion marks of exg_arena_mark, a ring
+    uint64_t marks_issued = 0;
     bool use_graph = false;
     int settings_epoch = 0;   // bumped by anything baked into captured kernel arguments
     struct Graph {
@@ -137,8 +143,11 @@ struct exg_arena {
         for (auto& c : class_ids) c.release();
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : marks)
+            if (e) (void)hipEventDestroy(e);
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        classes.destroy();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -155,7 +164,9 @@ struct exg_arena {
     // Gravity work table and island-class id lists of the live worlds; with
     // `override_chunk` ≥ 0 every world is chunked by that setting instead of its own
     // (timing only: exg_arena_autotune_gravity).
-    int rebuild(int override_chunk = -1) {
+    // Work tables and class lists of the live worlds; override_chunk / override_mapping
+    // (≥ 0) replace every world's own setting (the autotuners' trials).
+    int rebuild(int override_chunk = -1, int override_mapping = -1) {
         std::vector<exg::GravEntry> en;
         std::vector<exg::GravTile> tl;
         std::vector<int> ids[exg::ISLAND_CLASSES];
@@ -176,7 +187,12 @@ struct exg_arena {
                     tl.push_back(z);
                 }
             }
-            for (long long i = w.i0; i < w.i0 + w.ni; ++i) ids[island_class_of[(size_t)i]].push_back((int)i);
+            const int mapping = override_mapping >= 0 ? override_mapping : w.mapping;
+            for (long long i = w.i0; i < w.i0 + w.ni; ++i) {
+                int cls = island_class_of[(size_t)i];
+                if (cls == exg::ISLAND_CLASS_M8J14 && mapping == EXG_JOINT_MAPPING_FLAT) cls = exg::ISLAND_CLASS_M8J14_FLAT;
+                ids[cls].push_back((int)i);
+            }
         }
         if (en.size() > entries.n) {
             entries.release();
@@ -235,7 +251,9 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
     r->Xc = inverse_capacity;
     R_TRY(hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, 0));
     R_TRY(hipStreamCreate(&r->stream));
+    R_TRY(r->classes.create());
     for (auto& e : r->ev) R_TRY(hipEventCreate(&e));
+    for (auto& e : r->marks) R_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     const size_t B = (size_t)body_capacity, I = (size_t)island_capacity, J = (size_t)std::max(joint_capacity, 1);
     R_TRY(r->pose.alloc(B * 8));
     R_TRY(r->mom.alloc(B * 6));
@@ -378,6 +396,7 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     w.G = G;
     w.gravity_on = gravity_on != 0;
     w.chunk = r->chunk_for(n_bodies, r->grav_chunk);
+    w.mapping = r->joint_mapping;
     const size_t B = (size_t)n_bodies, I = (size_t)n_islands, J = (size_t)n_joints;
     auto up = [&](void* dst, const void* src, size_t bytes) { return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice); };
     A_TRY(up(r->pose.p + w.b0 * 8, pose, B * 8 * sizeof(double)));
@@ -491,13 +510,10 @@ static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
     a.ns_seed = r->ns_seed;
     a.ns_engine = r->ns_engine;
     a.jac_f32 = r->jac_prec == EXG_PREC_F32_COMPENSATED;
-    for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
-        if (!r->class_n[c]) continue;
-        a.island_ids = r->class_ids[c].p;
-        a.n_islands = r->class_n[c];
-        exg::launch_island_class(a, c, r->stream);
-    }
-    A_TRY(hipGetLastError());
+    int* ids[exg::ISLAND_CLASSES];
+    for (int c = 0; c < exg::ISLAND_CLASSES; ++c) ids[c] = r->class_ids[c].p;
+    const exg::ClassLaunch cl{r->class_n, ids, r->stream, r->classes.side, r->classes.fork, r->classes.join};
+    A_TRY(exg::launch_classes(a, cl));
     r->front = 1 - r->front;
     return EXG_OK;
 }
@@ -596,9 +612,139 @@ int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
     return EXG_OK;
 }
 
+int exg_arena_set_joint_mapping(exg_arena* r, int mapping) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_joint_mapping: null arena");
+    if (mapping != EXG_JOINT_MAPPING_SPLIT && mapping != EXG_JOINT_MAPPING_FLAT)
+        return afail(EXG_ERR_INVALID, "exg_arena_set_joint_mapping: unknown mapping " + std::to_string(mapping));
+    r->joint_mapping = mapping;   // for worlds admitted from now on
+    return EXG_OK;
+}
+
+int exg_arena_autotune_joints(exg_arena* r, int* chosen_mapping) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_autotune_joints: null arena");
+    if (r->dirty)
+        if (int rc = r->rebuild()) return rc;
+    bool any = false;
+    for (auto& w : r->worlds) any |= w.live;
+    if (!any) {
+        if (chosen_mapping) *chosen_mapping = r->joint_mapping;
+        return EXG_OK;
+    }
+    A_TRY(hipStreamSynchronize(r->stream));
+    // what a step writes, saved and put back after every trial step (the mappings
+    // give the same results, so the arena's worlds end where they started)
+    struct Saved {
+        void* dev;
+        size_t bytes;
+        void* copy;
+    };
+    std::vector<Saved> saved;
+    auto keep = [&](void* dev, size_t bytes) { saved.push_back({dev, bytes, nullptr}); };
+    keep(r->pose.p, r->pose.n * sizeof(double));
+    keep(r->mom.p, r->mom.n * sizeof(double));
+    keep(r->origin.p, r->origin.n * sizeof(double));
+    keep(r->xacc.p, r->xacc.n * sizeof(double));
+    keep(r->charges[0].p, r->charges[0].n * sizeof(double));
+    keep(r->charges[1].p, r->charges[1].n * sizeof(double));
+    keep(r->seeded.p, r->seeded.n * sizeof(int));
+    keep(r->stats.p, r->stats.n * sizeof(int));
+    keep(r->fatal.p, r->fatal.n * sizeof(int));
+    auto release = [&]() {
+        for (auto& x : saved)
+            if (x.copy) (void)hipFree(x.copy);
+    };
+    for (auto& x : saved) {
+        if (!x.bytes) continue;
+        hipError_t e = hipMalloc(&x.copy, x.bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(x.copy, x.dev, x.bytes, hipMemcpyDeviceToDevice, r->stream);
+        if (e != hipSuccess) {
+            release();
+            return afail(EXG_ERR_BACKEND, std::string("exg_arena_autotune_joints: ") + hipGetErrorString(e));
+        }
+    }
+    const int front0 = r->front, mapping0 = r->joint_mapping;
+    auto restore = [&]() -> hipError_t {
+        for (auto& x : saved)
+            if (x.bytes) {
+                const hipError_t e = hipMemcpyAsync(x.dev, x.copy, x.bytes, hipMemcpyDeviceToDevice, r->stream);
+                if (e != hipSuccess) return e;
+            }
+        r->front = front0;
+        return hipSuccess;
+    };
+    // each mapping: one warm step, then the best of three timed steps, each from the
+    // saved state, at the step of the mixed-workload benchmarks (h = 1e-2)
+    static const int candidates[] = {EXG_JOINT_MAPPING_SPLIT, EXG_JOINT_MAPPING_FLAT};
+    int best = mapping0;
+    float best_ms = -1.0f;
+    int rc = EXG_OK;
+    for (int c : candidates) {
+        rc = r->rebuild(-1, c);   // every live world's 8-body islands on mapping c
+        for (int rep = 0; rep < 4 && rc == EXG_OK; ++rep) {
+            hipError_t e = restore();
+            if (e == hipSuccess) e = hipEventRecord(r->ev[0], r->stream);
+            if (e == hipSuccess) rc = enqueue_arena_step(r, 1e-2, 1.0);
+            if (rc == EXG_OK && e == hipSuccess) e = hipEventRecord(r->ev[1], r->stream);
+            if (rc == EXG_OK && e == hipSuccess) e = hipEventSynchronize(r->ev[1]);
+            float ms = 0.0f;
+            if (rc == EXG_OK && e == hipSuccess) e = hipEventElapsedTime(&ms, r->ev[0], r->ev[1]);
+            if (e != hipSuccess) rc = afail(EXG_ERR_BACKEND, std::string("exg_arena_autotune_joints: ") + hipGetErrorString(e));
+            if (rc == EXG_OK && rep > 0 && (best_ms < 0.0f || ms < best_ms)) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (rc != EXG_OK) break;
+    }
+    const hipError_t e = restore();
+    const hipError_t es = hipStreamSynchronize(r->stream);
+    release();
+    if (int rc2 = r->rebuild()) return rc2;   // the worlds' own mappings again
+    if (rc != EXG_OK) return rc;
+    if (e != hipSuccess || es != hipSuccess)
+        return afail(EXG_ERR_BACKEND, std::string("exg_arena_autotune_joints: ") +
+                                          hipGetErrorString(e != hipSuccess ? e : es));
+    if (int rc2 = exg_arena_set_joint_mapping(r, best)) return rc2;
+    if (chosen_mapping) *chosen_mapping = best;
+    return EXG_OK;
+}
+
 int exg_arena_set_graph(exg_arena* r, int enable) {
     if (!r) return afail(EXG_ERR_INVALID, "exg_arena_set_graph: null arena");
     r->use_graph = enable != 0;
+    return EXG_OK;
+}
+
+int exg_arena_mark(exg_arena* r, uint64_t* ticket) {
+    if (!r || !ticket) return afail(EXG_ERR_INVALID, "exg_arena_mark: null argument");
+    const uint64_t t = r->marks_issued;
+    hipEvent_t e = r->marks[t % exg_arena::N_MARKS];
+    if (t >= (uint64_t)exg_arena::N_MARKS) A_TRY(hipEventSynchronize(e));   // the ticket it carried has passed
+    A_TRY(hipEventRecord(e, r->stream));
+    r->marks_issued = t + 1;
+    *ticket = t;
+    return EXG_OK;
+}
+
+int exg_arena_mark_done(exg_arena* r, uint64_t ticket, int* done) {
+    if (!r || !done) return afail(EXG_ERR_INVALID, "exg_arena_mark_done: null argument");
+    if (ticket >= r->marks_issued) return afail(EXG_ERR_INVALID, "exg_arena_mark_done: ticket not issued");
+    if (ticket + exg_arena::N_MARKS <= r->marks_issued) {
+        *done = 1;   // its ring slot was reused, which waited for it
+        return EXG_OK;
+    }
+    const hipError_t e = hipEventQuery(r->marks[ticket % exg_arena::N_MARKS]);
+    if (e == hipSuccess) *done = 1;
+    else if (e == hipErrorNotReady) *done = 0;
+    else return afail(EXG_ERR_BACKEND, std::string("exg_arena_mark_done: ") + hipGetErrorString(e));
+    return EXG_OK;
+}
+
+int exg_arena_mark_wait(exg_arena* r, uint64_t ticket) {
+    if (!r) return afail(EXG_ERR_INVALID, "exg_arena_mark_wait: null arena");
+    if (ticket >= r->marks_issued) return afail(EXG_ERR_INVALID, "exg_arena_mark_wait: ticket not issued");
+    if (ticket + exg_arena::N_MARKS <= r->marks_issued) return EXG_OK;
+    A_TRY(hipEventSynchronize(r->marks[ticket % exg_arena::N_MARKS]));
     return EXG_OK;
 }
 

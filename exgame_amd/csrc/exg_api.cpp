@@ -129,10 +129,12 @@ struct exg_world {
     double tol2 = 1e-8;
     int ns_warm = 2, ns_seed = 24;
     int ns_engine = EXG_NS_MFMA;
+    int joint_mapping = EXG_JOINT_MAPPING_SPLIT;
     int max_m = 0, max_j = 0;
     DevBuf<int> class_ids[exg::ISLAND_CLASSES];   // island ids routed to each kernel variant
     int class_n[exg::ISLAND_CLASSES] = {};
     hipStream_t stream = nullptr;
+    exg::ClassStreams classes;       // the island classes run side by side
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], grav_partial, grav_stage;
     DevBuf<exg::GravEntry> grav_entries;
     DevBuf<exg::GravTile> grav_tiles;
@@ -175,6 +177,7 @@ struct exg_world {
         ev.clear();
         for (auto& g : graphs)
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        classes.destroy();
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
     }
@@ -242,6 +245,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
         }                                                                                              \
     } while (0)
     W_TRY(hipStreamCreate(&w->stream));
+    W_TRY(w->classes.create());
     const size_t B = n_bodies, I = n_islands, Jn = n_joints;
     W_TRY(w->pose.alloc(B * 8));
     W_TRY(w->mom.alloc(B * 6));
@@ -401,6 +405,15 @@ int exg_world_set_ns_engine(exg_world* w, int engine) {
     return EXG_OK;
 }
 
+int exg_world_set_joint_mapping(exg_world* w, int mapping) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_joint_mapping: null world");
+    if (mapping != EXG_JOINT_MAPPING_SPLIT && mapping != EXG_JOINT_MAPPING_FLAT)
+        return fail(EXG_ERR_INVALID, "exg_world_set_joint_mapping: unknown mapping " + std::to_string(mapping));
+    w->joint_mapping = mapping;
+    w->settings_epoch++;
+    return EXG_OK;
+}
+
 // Enqueue one world step on the world's stream (no event records: also used inside
 // graph capture).
 static int enqueue_world_step(exg_world* w, double dt, double warp) {
@@ -457,19 +470,22 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.n_islands = w->I;
     hipEvent_t i0 = ev ? ev[2] : nullptr, i1 = ev ? ev[3] : nullptr;
     if (w->integrator == EXG_NEWTON) {
-        int first = -1, last = -1;
-        for (int c = 0; c < exg::ISLAND_CLASSES; ++c)
-            if (w->class_n[c]) {
-                if (first < 0) first = c;
-                last = c;
-            }
+        int* ids[exg::ISLAND_CLASSES];
+        int n[exg::ISLAND_CLASSES];
+        bool any = false;
         for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
-            if (!w->class_n[c]) continue;
-            a.island_ids = w->class_ids[c].p;
-            a.n_islands = w->class_n[c];
-            exg::launch_island_class(a, c, w->stream, c == first ? i0 : nullptr, c == last ? i1 : nullptr);
+            ids[c] = w->class_ids[c].p;
+            n[c] = w->class_n[c];
+            any |= n[c] > 0;
         }
-        if (ev && first < 0) {
+        if (w->joint_mapping == EXG_JOINT_MAPPING_FLAT) {   // the m8j14 islands on the other mapping
+            ids[exg::ISLAND_CLASS_M8J14_FLAT] = ids[exg::ISLAND_CLASS_M8J14];
+            n[exg::ISLAND_CLASS_M8J14_FLAT] = n[exg::ISLAND_CLASS_M8J14];
+            n[exg::ISLAND_CLASS_M8J14] = 0;
+        }
+        const exg::ClassLaunch cl{n, ids, w->stream, w->classes.side, w->classes.fork, w->classes.join};
+        HIP_TRY(exg::launch_classes(a, cl, i0, i1));
+        if (ev && !any) {
             HIP_TRY(hipEventRecord(i0, w->stream));
             HIP_TRY(hipEventRecord(i1, w->stream));
         }

@@ -116,9 +116,12 @@ __device__ inline void mark_fatal(const IslandArgs& a, int isl) {
 
 // Island classes of the fused Newton kernel and the launch of one class over the
 // island ids in a.island_ids[0 .. a.n_islands).
+// ISLAND_CLASS_M8J14_FLAT runs the islands of the m8j14 class with the other joint
+// mapping (EXG_JOINT_MAPPING_FLAT): island_class never returns it; a world or an
+// arena world that asks for that mapping routes its m8j14 islands there.
 enum {
     ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASS_LARGE = 4,
-    ISLAND_CLASSES = 5
+    ISLAND_CLASS_M8J14_FLAT = 5, ISLAND_CLASSES = 6
 };
 int island_class(int m, int joints);   // -1 when no variant holds the island
 // Waves per workgroup and MFMA tiles per wave of a class's kernel (for its tile lists).
@@ -139,6 +142,31 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t stream, hipEve
 // (hipExtLaunchKernelGGL: no marker packets between kernels); null records nothing.
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start = nullptr,
                          hipEvent_t stop = nullptr);
+// Every island class of one step, run side by side: the classes with islands are
+// launched heaviest first (large, m8j14, m8j14 flat, m8j16, m4j8, singles), the first on `main`
+// and each further one on its own side stream after `fork` (recorded on main), and
+// `main` then waits for `join[c]` of each. Classes hold disjoint islands, so their
+// order does not change any result. `class_ids[c]` lists class c's islands. With a
+// single class, `start`/`stop` are recorded by its dispatch; with several, `start` by
+// the first dispatch and `stop` on main after the joins. Also valid under stream
+// capture (the fork and joins become graph edges).
+struct ClassLaunch {
+    const int* n;               // islands per class [ISLAND_CLASSES]
+    int* const* ids;            // island ids per class
+    hipStream_t main;
+    const hipStream_t* side;    // one per class
+    hipEvent_t fork;
+    const hipEvent_t* join;     // one per class
+};
+hipError_t launch_classes(IslandArgs a, const ClassLaunch& c, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// The side streams (default priority) and fork/join events (no timing) of launch_classes.
+struct ClassStreams {
+    hipStream_t side[ISLAND_CLASSES] = {};
+    hipEvent_t fork = nullptr, join[ISLAND_CLASSES] = {};
+    hipError_t create();
+    void destroy();
+};
+
 // Mechanism-sized islands, one workgroup of LARGE_THREADS per island (island_large.hip).
 constexpr int LARGE_THREADS = 512;
 void launch_island_large(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);

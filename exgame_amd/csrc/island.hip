@@ -125,7 +125,7 @@ __device__ double block_sum(IslandShared<MAXM, MAXJ, NTH, JF>& s, double v) {
     return uni(r);
 }
 
-template <int MAXM, int MAXJ, int NTH, bool JF>
+template <int MAXM, int MAXJ, int NTH, bool JF, bool FLAT = false>
 struct IslandSolver {
     IslandShared<MAXM, MAXJ, NTH, JF>& s;
     const IslandArgs& a;
@@ -135,7 +135,10 @@ struct IslandSolver {
     // candidate (one wave) and four or more waves, three candidates take a wave each
     // and the fourth wave evaluates their joints (3·MAXJ ≤ 64 items); otherwise LS
     // candidates per round in three phases (pre_candidates, joints_plain, gather_post).
-    static constexpr bool SPLIT = NTH / 64 >= 4 && 8 * MAXM == 64 && 3 * MAXJ <= 64;
+    // (FLAT keeps LS candidates per round on any shape and every velocity column its
+    // own lane: the same results, another mapping of the joint work onto the waves,
+    // for the joint-mapping autotuner)
+    static constexpr bool SPLIT = !FLAT && NTH / 64 >= 4 && 8 * MAXM == 64 && 3 * MAXJ <= 64;
     static constexpr int LSR = SPLIT ? 3 : LS;
     int m, J, isl, first_body;
     int n, n16;               // system size 6m and its 16-padded size
@@ -220,7 +223,7 @@ struct IslandSolver {
     __device__ void joints_jacobian() {
         if constexpr (MAXJ == 0) return;
         const int P = J * 12, t = tid();
-        if (P + 6 * J <= NTH) {
+        if (!FLAT && P + 6 * J <= NTH) {
             // one column per thread: the pose columns, then the velocity columns of
             // end a, each also giving end b's column by negation (vrel = va − vb enters
             // the wrenches linearly, so ∂W/∂vb = −∂W/∂va exactly)
@@ -1023,7 +1026,7 @@ struct IslandSolver {
     }
 };
 
-template <int MAXM, int MAXJ, int NTH, bool JF>
+template <int MAXM, int MAXJ, int NTH, bool JF, bool FLAT = false>
 __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH, JF>& s) {
     if ((int)blockIdx.x >= a.n_islands) return;
     const int isl = a.island_ids ? a.island_ids[blockIdx.x] : (int)blockIdx.x;
@@ -1091,7 +1094,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         s.nbr_row[threadIdx.x] = (unsigned char)i;
     }
 
-    IslandSolver<MAXM, MAXJ, NTH, JF> sv{s, a};
+    IslandSolver<MAXM, MAXJ, NTH, JF, FLAT> sv{s, a};
     sv.m = m;
     sv.J = J;
     sv.n = n;
@@ -1182,6 +1185,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void i
     __shared__ IslandShared<8, 14, 256, false> s;
     island_step<8, 14, 256, false>(a, s);
 }
+// the same step with four line-search candidates per round, the joints on every wave
+// and one Jacobian column per lane (the same results to rounding: the fused
+// multiply-adds fall differently; selected per world by its joint mapping)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14_flat(IslandArgs a) {
+    __shared__ IslandShared<8, 14, 256, false> s;
+    island_step<8, 14, 256, false, true>(a, s);
+}
 __global__ __launch_bounds__(256) void island_step_kernel_m8j16(IslandArgs a) {
     __shared__ IslandShared<8, 16, 256, false> s;
     island_step<8, 16, 256, false>(a, s);
@@ -1195,6 +1205,10 @@ __global__ __launch_bounds__(128) void island_step_kernel_m4j8_jf32(IslandArgs a
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14_jf32(IslandArgs a) {
     __shared__ IslandShared<8, 14, 256, true> s;
     island_step<8, 14, 256, true>(a, s);
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14_jf32_flat(IslandArgs a) {
+    __shared__ IslandShared<8, 14, 256, true> s;
+    island_step<8, 14, 256, true, true>(a, s);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j16_jf32(IslandArgs a) {
     __shared__ IslandShared<8, 16, 256, true> s;
@@ -1217,6 +1231,7 @@ void island_class_shape(int cls, int& waves, int& per_wave) {
             per_wave = IslandShared<4, 8, 128, false>::PER_WAVE;
             break;
         case ISLAND_CLASS_M8J14:
+        case ISLAND_CLASS_M8J14_FLAT:
             waves = IslandShared<8, 14, 256, false>::WAVES;
             per_wave = IslandShared<8, 14, 256, false>::PER_WAVE;
             break;
@@ -1227,6 +1242,55 @@ void island_class_shape(int cls, int& waves, int& per_wave) {
         default:
             waves = per_wave = 1;
     }
+}
+
+hipError_t ClassStreams::create() {
+    hipError_t e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    for (int k = 0; k < ISLAND_CLASSES && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&side[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&join[k], hipEventDisableTiming);
+    }
+    return e;
+}
+void ClassStreams::destroy() {
+    for (int k = 0; k < ISLAND_CLASSES; ++k) {
+        if (join[k]) (void)hipEventDestroy(join[k]);
+        if (side[k]) (void)hipStreamDestroy(side[k]);
+        join[k] = nullptr;
+        side[k] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
+}
+
+hipError_t launch_classes(IslandArgs a, const ClassLaunch& c, hipEvent_t start, hipEvent_t stop) {
+    static const int order[ISLAND_CLASSES] = {ISLAND_CLASS_LARGE, ISLAND_CLASS_M8J14, ISLAND_CLASS_M8J14_FLAT,
+                                              ISLAND_CLASS_M8J16, ISLAND_CLASS_M4J8, ISLAND_CLASS_M1};
+    int live[ISLAND_CLASSES], nl = 0;
+    for (int k = 0; k < ISLAND_CLASSES; ++k)
+        if (c.n[order[k]] > 0) live[nl++] = order[k];
+    if (nl == 0) return hipSuccess;
+    if (nl == 1) {
+        a.island_ids = c.ids[live[0]];
+        a.n_islands = c.n[live[0]];
+        launch_island_class(a, live[0], c.main, start, stop);
+        return hipGetLastError();
+    }
+    hipError_t e = hipEventRecord(c.fork, c.main);
+    for (int k = 1; k < nl && e == hipSuccess; ++k) e = hipStreamWaitEvent(c.side[live[k]], c.fork, 0);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < nl; ++k) {
+        a.island_ids = c.ids[live[k]];
+        a.n_islands = c.n[live[k]];
+        launch_island_class(a, live[k], k == 0 ? c.main : c.side[live[k]], k == 0 ? start : nullptr, nullptr);
+    }
+    e = hipGetLastError();
+    for (int k = 1; k < nl && e == hipSuccess; ++k) {
+        e = hipEventRecord(c.join[live[k]], c.side[live[k]]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(c.main, c.join[live[k]], 0);
+    }
+    if (e == hipSuccess && stop) e = hipEventRecord(stop, c.main);
+    return e;
 }
 
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
@@ -1243,6 +1307,10 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEv
         case ISLAND_CLASS_M8J14:
             if (a.jac_f32) launch_k(island_step_kernel_m8j14_jf32, grid, dim3(256), stream, start, stop, a);
             else launch_k(island_step_kernel_m8j14, grid, dim3(256), stream, start, stop, a);
+            break;
+        case ISLAND_CLASS_M8J14_FLAT:
+            if (a.jac_f32) launch_k(island_step_kernel_m8j14_jf32_flat, grid, dim3(256), stream, start, stop, a);
+            else launch_k(island_step_kernel_m8j14_flat, grid, dim3(256), stream, start, stop, a);
             break;
         case ISLAND_CLASS_LARGE:
             launch_island_large(a, stream, start, stop);

@@ -81,58 +81,92 @@ class Server:
         self.tick = int(tick_steps)
         self.keep = keep_results
 
+    # ticks kept queued on the device while the host waits for the oldest one
+    DEPTH = 2
+
     def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
         """Serve `requests` (any order); returns a report dict. Latency is completion
-        time minus arrival time on `clock`."""
+        time minus arrival time on `clock`.
+
+        Ticks are pipelined: while the set of worlds stays the same, the next tick is
+        enqueued before the previous one has finished (up to DEPTH in flight, waited
+        for by completion marks), so the device does not idle between ticks. The set
+        changes — worlds retire when their last tick has run, waiting requests are
+        admitted — only with nothing in flight, since admission rebuilds the arena's
+        tables."""
         arrivals = sorted(requests, key=lambda r: r.arrival)
         queue: list[Request] = []
-        active: dict[int, list] = {}       # world id → [request, steps left, bytes]
+        active: dict[int, list] = {}       # world id → [request, steps not yet enqueued, bytes]
+        inflight: list[int] = []           # completion-mark tickets, oldest first
         used = 0
         body_steps = 0
         nxt = 0
         t0 = clock()
         busy = 0.0
+        busy_from = None
         done = []
-        while nxt < len(arrivals) or queue or active:
+        full = False          # the arena refused the queue's head; retry after a retirement
+
+        def fits(r):
+            return not full and (used + footprint(r.scene, self.arena.gravity_chunk) <= self.budget or not active)
+
+        def drain():
+            nonlocal busy, busy_from
+            if inflight:
+                self.arena.sync()
+                inflight.clear()
+            if busy_from is not None:
+                busy += clock() - busy_from
+                busy_from = None
+
+        while nxt < len(arrivals) or queue or active or inflight:
             now = clock() - t0
             while nxt < len(arrivals) and arrivals[nxt].arrival <= now:
                 queue.append(arrivals[nxt])
                 nxt += 1
-            # admission in arrival order, by HBM footprint
-            while queue:
-                r = queue[0]
-                need = footprint(r.scene, self.arena.gravity_chunk)
-                if used + need > self.budget and active:
-                    break
-                wid = self.arena.admit(r.scene)
-                if wid is None:
-                    if not active:
-                        raise RuntimeError(f"request {r.rid} ({r.scene.n_bodies} bodies) does not fit the arena")
-                    break
-                queue.pop(0)
-                r.admitted = clock() - t0
-                active[wid] = [r, r.steps, need]
-                used += need
+            finishing = [w for w, v in active.items() if v[1] <= 0]
+            if finishing or (queue and fits(queue[0])):
+                drain()
+                for wid in finishing:
+                    full = False
+                    r, _, need = active.pop(wid)
+                    if self.keep:
+                        r.result = self.arena.state(wid)
+                    self.arena.retire(wid)
+                    used -= need
+                    r.finished = clock() - t0
+                    done.append(r)
+                # admission in arrival order, by HBM footprint
+                while queue and fits(queue[0]):
+                    r = queue[0]
+                    need = footprint(r.scene, self.arena.gravity_chunk)
+                    wid = self.arena.admit(r.scene)
+                    if wid is None:
+                        if not active:
+                            raise RuntimeError(f"request {r.rid} ({r.scene.n_bodies} bodies) does not fit the arena")
+                        full = True
+                        break
+                    queue.pop(0)
+                    r.admitted = clock() - t0
+                    active[wid] = [r, r.steps, need]
+                    used += need
             if not active:
+                drain()
                 if nxt < len(arrivals):
                     sleep(max(0.0, arrivals[nxt].arrival - (clock() - t0)))
                 continue
-            k = min(self.tick, min(v[1] for v in active.values()))
-            ts = clock()
-            self.arena.step(k, self.dt, 1.0)
-            self.arena.sync()
-            busy += clock() - ts
-            body_steps += k * sum(v[0].scene.n_bodies for v in active.values())
-            for wid in [w for w, v in active.items() if v[1] - k <= 0]:
-                r, _, need = active.pop(wid)
-                if self.keep:
-                    r.result = self.arena.state(wid)
-                self.arena.retire(wid)
-                used -= need
-                r.finished = clock() - t0
-                done.append(r)
-            for v in active.values():
-                v[1] -= k
+            if len(inflight) < self.DEPTH:
+                k = min(self.tick, min(v[1] for v in active.values()))
+                self.arena.step(k, self.dt, 1.0)
+                inflight.append(self.arena.mark())
+                if busy_from is None:
+                    busy_from = clock()
+                body_steps += k * sum(v[0].scene.n_bodies for v in active.values())
+                for v in active.values():
+                    v[1] -= k
+                continue
+            self.arena.mark_wait(inflight.pop(0))
+        drain()
         wall = clock() - t0
         lat = np.array([r.latency for r in done])
         return {

@@ -28,10 +28,14 @@ class Epoch:
     warp: float = 1.0
 
 
+# how an 8-body island's joint work is spread over its waves (bit-identical results)
+JOINT_MAPPINGS = {"split": 0, "flat": 1}
+
+
 class DeviceWorld:
     def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
-                 jacobian_precision: str = "f64", graph: bool = False):
+                 jacobian_precision: str = "f64", graph: bool = False, joint_mapping: str = "split"):
         L = _lib.lib()
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
@@ -65,6 +69,7 @@ class DeviceWorld:
         _lib.check(L.exg_world_set_ns_engine(self.h, NS_ENGINES[ns_engine]))
         _lib.check(L.exg_world_set_precision(self.h, PRECISIONS[gravity_precision], PRECISIONS[jacobian_precision]))
         _lib.check(L.exg_world_set_graph(self.h, int(bool(graph))))
+        _lib.check(L.exg_world_set_joint_mapping(self.h, JOINT_MAPPINGS[joint_mapping]))
 
     def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
         if isinstance(dt, Epoch):

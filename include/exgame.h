@@ -112,6 +112,17 @@ int exg_world_set_solver(exg_world* w, int integrator, double tol2, int ns_warm,
 enum { EXG_NS_MFMA = 0, EXG_NS_VALU_ORDERED = 1 };
 int exg_world_set_ns_engine(exg_world* w, int engine);
 
+/* How the joint work of an 8-body island's step is spread over its waves (no
+ * counterpart in the reference): EXG_JOINT_MAPPING_SPLIT (default) evaluates three
+ * line-search candidates a round, one wave each, beside a wave that evaluates their
+ * joints, and forms end b's velocity Jacobian columns by negation of end a's;
+ * EXG_JOINT_MAPPING_FLAT evaluates four candidates a round with the joints on every
+ * wave, one Jacobian column per lane. The same arithmetic in another order of
+ * instructions: results agree to rounding (the compiler fuses multiply-adds
+ * differently), not bit for bit. */
+enum { EXG_JOINT_MAPPING_SPLIT = 0, EXG_JOINT_MAPPING_FLAT = 1 };
+int exg_world_set_joint_mapping(exg_world* w, int mapping);
+
 /* Advance `n_steps` steps of length dt > 0 with time warp `warp` ≥ 1 (aristotle Epoch dt/warp,
  * crates/aristotle/src/epoch.rs:238-248): per step, gravity from the charges published
  * by the previous step (zero on the first, gravity.rs:620-626), the island solve, the
@@ -284,17 +295,29 @@ int exg_arena_destroy(exg_arena* r);
 int exg_arena_set_solver(exg_arena* r, double tol2, int ns_warm, int ns_seed, int ns_engine, int gravity_precision);
 /* Precision of gravity and of the Jacobian assembly for every world (EXG_PREC_*). */
 int exg_arena_set_precision(exg_arena* r, int gravity, int jacobian);
-/* Gravity work split: sources per chunk (≥ 64), the same for every world. */
+/* Gravity work split of worlds admitted from now on: sources per chunk (≥ 64), or 0
+ * for the single-world rule (chunks sized to fill the device for that world's body
+ * count). A world keeps the chunking it was admitted with, so its forces do not
+ * depend on the other worlds. */
 int exg_arena_set_gravity_chunk(exg_arena* r, int chunk_len);
 /* Replay steps from a captured HIP graph of two consecutive steps of every live
  * world (re-captured when dt, warp, a setting or the set of worlds changes). No
  * counterpart in the reference (its Driver records command buffers per flush). */
 int exg_arena_set_graph(exg_arena* r, int enable);
-/* Time the gravity pass of the live worlds for chunk lengths 128 … 8192 and keep the
- * fastest (written to *chosen_chunk when not null). The chunking sets the order of
- * each world's force sums, so results are bit-identical across arenas only at equal
- * chunk lengths. No counterpart in the reference. */
+/* Time the gravity pass of the live worlds with every world chunked by 0 (the
+ * single-world rule), 128 … 8192, and make the fastest the setting for worlds
+ * admitted from now on (written to *chosen_chunk when not null); the live worlds keep
+ * their chunking. No counterpart in the reference. */
 int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk);
+/* The joint mapping (EXG_JOINT_MAPPING_*) of worlds admitted from now on; a world
+ * keeps the mapping it was admitted with, so its results do not depend on the others. */
+int exg_arena_set_joint_mapping(exg_arena* r, int mapping);
+/* Time one step of the live worlds with all of them on each joint mapping (every
+ * trial from the same saved state, which is put back: the worlds do not advance and
+ * keep their own mappings) and make the fastest the mapping of later admissions
+ * (written to *chosen_mapping when not null). Allocates a copy of the state for the
+ * duration of the call. No counterpart in the reference. */
+int exg_arena_autotune_joints(exg_arena* r, int* chosen_mapping);
 /* Admit a world (arrays as for exg_world_create); EXG_ERR_FULL when no range fits. */
 int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double* momentum, const double* mass,
                     const double* inertia, const double* size, int n_islands, const int* island_start,
@@ -305,6 +328,15 @@ int exg_arena_retire(exg_arena* r, int world_id);
 int exg_arena_step(exg_arena* r, int n_steps, double dt, double warp);
 /* As exg_world_sync, EXG_ERR_FATAL for non-finite island-steps of any world. */
 int exg_arena_sync(exg_arena* r);
+/* Completion marks, so that a host loop can keep steps queued while it waits for
+ * earlier ones (no counterpart in the reference): exg_arena_mark puts a mark behind
+ * the work enqueued so far and returns its ticket (consecutive from 0);
+ * exg_arena_mark_done tells without blocking whether that work has finished,
+ * exg_arena_mark_wait blocks until it has. Marks do not check fatals (exg_arena_sync
+ * does). */
+int exg_arena_mark(exg_arena* r, uint64_t* ticket);
+int exg_arena_mark_done(exg_arena* r, uint64_t ticket, int* done);
+int exg_arena_mark_wait(exg_arena* r, uint64_t ticket);
 /* One world's state: pose[n*8], momentum[n*6], origin[islands*3]; any may be NULL. */
 int exg_arena_download(exg_arena* r, int world_id, double* pose, double* momentum, double* origin);
 /* One world's per-island solver counters, islands*8 (as exg_world_stats). */

@@ -40,7 +40,7 @@ def _resources(src: str, tmp_path) -> dict:
 @pytest.mark.skipif(shutil.which("hipcc") is None, reason="hipcc not on PATH")
 def test_island_kernels_use_no_scratch(tmp_path):
     kernels = _resources("island.hip", tmp_path)
-    assert len(kernels) == 6, kernels
+    assert len(kernels) == 8, kernels
     assert all(v[0] == 0 for v in kernels.values()), kernels
 
 

@@ -35,7 +35,14 @@ class FakeArena:
         self.log.append(("step", k, tuple(sorted(self.live))))
 
     def sync(self):
-        pass
+        self.log.append(("sync",))
+
+    def mark(self):
+        self.log.append(("mark",))
+        return sum(1 for e in self.log if e[0] == "mark") - 1
+
+    def mark_wait(self, ticket):
+        self.log.append(("wait", ticket))
 
     def state(self, wid):
         return {"steps": self.steps[wid]}
@@ -93,3 +100,22 @@ def test_hbm_budget_limits_admission_and_latency_report():
     assert abs(rep["latency_p50_s"] - np.percentile(lat, 50)) < 1e-12
     assert abs(rep["latency_p99_s"] - np.percentile(lat, 99)) < 1e-12
     assert rep["body_steps"] == sum(r.steps * r.scene.n_bodies for r in reqs)
+
+
+def test_ticks_stay_queued_while_the_set_of_worlds_is_unchanged():
+    # one long request: every tick is enqueued behind a completion mark, the host
+    # waits for the oldest of two in flight, and a full sync comes only when the
+    # world retires
+    clk = Clock()
+    fa = FakeArena(clk, max_worlds=4)
+    sc = scenes.jointed_chains(n_chains=2, m=8, seed=0)
+    rep = Server(fa, hbm_budget=1 << 40, tick_steps=10, keep_results=True).run([Request(0, sc, 95, 0.0)], clock=clk,
+                                                                             sleep=clk.sleep)
+    steps = [e for e in fa.log if e[0] == "step"]
+    assert [e[1] for e in steps] == [10] * 9 + [5]
+    syncs = [i for i, e in enumerate(fa.log) if e[0] == "sync"]
+    assert len(syncs) == 1 and syncs[0] > fa.log.index(steps[-1])
+    waits = [e for e in fa.log if e[0] == "wait"]
+    assert [e[1] for e in waits] == list(range(len(waits)))      # oldest first
+    assert len(waits) >= 8
+    assert rep["body_steps"] == 95 * sc.n_bodies
