@@ -110,6 +110,7 @@ struct exg_arena {
     DevBuf<exg::GravEntry> entries;
     DevBuf<exg::GravTile> tiles;
     size_t n_entries = 0, n_tiles = 0;
+    int stage_n = 0;   // bodies the mixed-precision source stage covers: up to the last live world's end
     DevBuf<int> class_ids[exg::ISLAND_CLASSES];
     int class_n[exg::ISLAND_CLASSES] = {};
     std::vector<int> island_class_of;   // per island slot
@@ -170,8 +171,10 @@ struct exg_arena {
         std::vector<exg::GravEntry> en;
         std::vector<exg::GravTile> tl;
         std::vector<int> ids[exg::ISLAND_CLASSES];
+        long long hw = 0;
         for (auto& w : worlds) {
             if (!w.live) continue;
+            hw = std::max(hw, w.b0 + w.nb);
             if (w.gravity_on) {
                 const int ch = override_chunk >= 0 ? chunk_for(w.nb, override_chunk) : w.chunk;
                 exg::gravity_add_world_chunked(en, tl, (int)w.b0, (int)w.nb, w.G, ch);
@@ -207,6 +210,7 @@ struct exg_arena {
         A_TRY(entries.upload(en.data(), en.size()));
         A_TRY(tiles.upload(tl.data(), tl.size()));
         n_entries = en.size();
+        stage_n = (int)hw;
         n_tiles = tl.size();
         for (int c = 0; c < exg::ISLAND_CLASSES; ++c) {
             class_n[c] = (int)ids[c].size();
@@ -476,7 +480,7 @@ static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
     double* front = r->charges[r->front].p;
     double* back = r->charges[1 - r->front].p;
     exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                        r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
+                        r->partial.p, r->ext.p, r->grav_stage.p, r->stage_n, r->stream);
     A_TRY(hipGetLastError());
     exg::IslandArgs a{};
     a.pose = r->pose.p;
@@ -591,11 +595,11 @@ int exg_arena_autotune_gravity(exg_arena* r, int* chosen_chunk) {
         if (int rc = r->rebuild(c)) return rc;
         double* front = r->charges[r->front].p;
         exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                            r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
+                            r->partial.p, r->ext.p, r->grav_stage.p, r->stage_n, r->stream);
         A_TRY(hipEventRecord(r->ev[0], r->stream));
         for (int k = 0; k < 5; ++k)
             exg::launch_gravity(front, r->entries.p, (int)r->n_entries, r->tiles.p, (int)r->n_tiles, r->grav_prec,
-                                r->partial.p, r->ext.p, r->grav_stage.p, r->Bc, r->stream);
+                                r->partial.p, r->ext.p, r->grav_stage.p, r->stage_n, r->stream);
         A_TRY(hipEventRecord(r->ev[1], r->stream));
         A_TRY(hipGetLastError());
         A_TRY(hipEventSynchronize(r->ev[1]));
