@@ -104,6 +104,8 @@ struct exg_arena {
     DevBuf<double> pose, mom, mass, tens, size, ext, origin, jparams, xacc, charges[2], partial, island_G, grav_stage;
     DevBuf<int> island_start, island_end, island_jstart, island_jend, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;                // island-steps that ended with a non-finite state
+    DevBuf<int> requeue;              // single-body islands whose step subdivides (capacity: every island)
+    DevBuf<unsigned> requeue_ctl;     // the requeue list's length and workgroup counter (0 between launches)
     long long fatal_reported = 0;
     DevBuf<long long> xacc_off;
     DevBuf<exg::IslandTopo> topo;
@@ -135,8 +137,9 @@ struct exg_arena {
                         &partial, &island_G, &grav_stage})
             b->release();
         for (auto* b : {&island_start, &island_end, &island_jstart, &island_jend, &jkind, &ja, &jb, &seeded, &stats,
-                        &fatal})
+                        &fatal, &requeue})
             b->release();
+        requeue_ctl.release();
         xacc_off.release();
         topo.release();
         entries.release();
@@ -278,6 +281,9 @@ int exg_arena_create(int body_capacity, int island_capacity, int joint_capacity,
     R_TRY(r->stats.alloc(I * exg::ISLAND_STATS));
     R_TRY(r->fatal.alloc(1));
     R_TRY(hipMemset(r->fatal.p, 0, sizeof(int)));
+    R_TRY(r->requeue.alloc(I));
+    R_TRY(r->requeue_ctl.alloc(2));
+    R_TRY(hipMemset(r->requeue_ctl.p, 0, 2 * sizeof(unsigned)));
     R_TRY(r->xacc_off.alloc(I));
     R_TRY(r->topo.alloc(I));
     R_TRY(r->xacc.alloc((size_t)inverse_capacity));
@@ -505,6 +511,8 @@ static int enqueue_arena_step(exg_arena* r, double dt, double warp) {
     a.charges_out = back;
     a.stats = r->stats.p;
     a.fatal = r->fatal.p;
+    a.requeue = r->requeue.p;
+    a.requeue_ctl = r->requeue_ctl.p;
     a.dt = dt;
     a.warp = warp;
     a.tol2 = r->tol2;
@@ -822,7 +830,7 @@ int exg_arena_footprint(int n_bodies, int n_islands, int n_joints, long long inv
     }
     const long long per_body = (8 + 6 + 1 + 9 + 1 + 6 + 16) * 8;                 // state, ext, two charges
     const long long per_island = 3 * 8 + 5 * 4 + exg::ISLAND_STATS * 4 + 8 +      // origin, ranges, stats, offset,
-                                 (long long)sizeof(exg::IslandTopo);             // topology record
+                                 (long long)sizeof(exg::IslandTopo) + 4;         // topology record, requeue slot
     const long long per_joint = 10 * 8 + 3 * 4;                                   // params, kind, ends
     const long long tiles = (n_bodies + 63) / 64;
     const long long chunks = (n_bodies + gravity_chunk - 1) / gravity_chunk;

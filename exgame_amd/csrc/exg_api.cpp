@@ -141,6 +141,8 @@ struct exg_world {
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;              // island-steps that ended with a non-finite state
+    DevBuf<int> requeue;            // single-body islands whose step subdivides (one per such island)
+    DevBuf<unsigned> requeue_ctl;   // the requeue list's length and workgroup counter (0 between launches)
     long long fatal_reported = 0;   // the count the last exg_world_sync reported
     DevBuf<long long> xacc_off;
     DevBuf<exg::IslandTopo> topo;   // per island, Newton integrator only
@@ -164,7 +166,8 @@ struct exg_world {
         for (auto* b : {&pose, &mom, &mass, &tens, &size, &ext, &origin, &jparams, &xacc, &charges[0], &charges[1],
                         &grav_partial, &grav_stage})
             b->release();
-        for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats, &fatal}) b->release();
+        for (auto* b : {&island_start, &island_jstart, &jkind, &ja, &jb, &seeded, &stats, &fatal, &requeue}) b->release();
+        requeue_ctl.release();
         grav_entries.release();
         grav_tiles.release();
         xacc_off.release();
@@ -286,6 +289,11 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
             if (ids[c].empty()) continue;
             W_TRY(w->class_ids[c].alloc(ids[c].size()));
             W_TRY(hipMemcpy(w->class_ids[c].p, ids[c].data(), ids[c].size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        if (w->class_n[exg::ISLAND_CLASS_M1] > 0) {
+            W_TRY(w->requeue.alloc(w->class_n[exg::ISLAND_CLASS_M1]));
+            W_TRY(w->requeue_ctl.alloc(2));
+            W_TRY(hipMemset(w->requeue_ctl.p, 0, 2 * sizeof(unsigned)));
         }
     }
     std::vector<long long> xoff(I);
@@ -459,6 +467,8 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.charges_out = back;
     a.stats = w->stats.p;
     a.fatal = w->fatal.p;
+    a.requeue = w->requeue.p;
+    a.requeue_ctl = w->requeue_ctl.p;
     a.dt = dt;
     a.warp = warp;
     a.tol2 = w->tol2;

@@ -105,6 +105,9 @@ struct IslandArgs {
     const LargeTopo* ltopo;   // per island, used by the large class only (others: unused entries)
     double* lws;              // workspace of the large islands
     const long long* lws_off; // per island: its offset into lws (large class only)
+    int* requeue;             // single-body islands whose step subdivides, for the requeue kernel (capacity:
+                              // the single-body islands of the launch); null: the one-lane kernel solves all
+    unsigned* requeue_ctl;    // [0] the list's length, [1] requeue workgroups done; both 0 between launches
 };
 
 // Mark-and-continue fatal report of one island-step (the state is written as it is):
@@ -170,7 +173,9 @@ struct ClassStreams {
 // Mechanism-sized islands, one workgroup of LARGE_THREADS per island (island_large.hip).
 constexpr int LARGE_THREADS = 512;
 void launch_island_large(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
-// Single-body islands without joints, one thread per island (singles.hip).
+// Single-body islands without joints (singles.hip): one lane per island, and, with a
+// requeue list, eight lanes per island for those whose step subdivides (a second launch
+// on the same stream).
 void launch_singles(const IslandArgs& a, hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // Explicit integrators (1 = SymplecticEuler, 0 = ExplicitEuler), one thread per island.

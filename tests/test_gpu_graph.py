@@ -69,3 +69,23 @@ def test_arena_graph_replays_equal_eager_steps_across_admit_and_retire():
         a, b = arenas[0].state(ids[0][w]), arenas[1].state(ids[1][w])
         for k in ("pose", "momentum", "origin"):
             assert np.array_equal(a[k], b[k]), (w, k)
+
+
+def test_graph_replays_with_subdividing_free_bodies():
+    # free bodies in close encounters (dense ball, dt = 1e-2): some steps subdivide, so
+    # those islands go through the requeue list to the eight-lane kernel inside each
+    # captured step; every replay must equal the eager steps bit for bit, and equal the
+    # eager steps of a world whose free bodies never take the requeue path
+    scene = scenes.plain_gravity(n=600, seed=4)
+    eager = DeviceWorld(scene, tol2=1e-20)
+    graph = DeviceWorld(scene, tol2=1e-20, graph=True)
+    for n in (4, 2, 5, 6):
+        eager.step(n, 1e-2, 1.0)
+        graph.step(n, 1e-2, 1.0)
+        a, b = eager.state(), graph.state()
+        for k in ("pose", "momentum", "origin"):
+            assert np.array_equal(a[k], b[k]), (n, k)
+        assert np.array_equal(eager.charges(), graph.charges())
+    st = eager.stats()
+    assert int(st[:, 0].max()) > 17, "the scene should subdivide some steps"
+    np.testing.assert_array_equal(st, graph.stats())
