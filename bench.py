@@ -75,6 +75,12 @@ def matvec_flops(scene) -> np.ndarray:
     return 2 * (6 * m) ** 2
 
 
+def stage(msg: str) -> None:
+    """One progress line on stderr as each part of the run finishes (stdout carries only
+    the JSON line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model() -> str:
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
@@ -376,23 +382,29 @@ def main() -> int:
                        "kernels_ms_per_step": {"gravity": rh["timing"]["gravity_ms"] / ERROR_HORIZON,
                                                "island_solve": rh["timing"]["island_ms"] / ERROR_HORIZON}}
     line["mixed_precision"] = mixed
+    stage(f"headline {value:.4g} body-steps/s, mixed precision done")
     if not args.headline_only:
         try:
             line["mixed_precision"]["vs_oracle"] = oracle_mixed_check()
         except Exception as e:  # reported beside the result, never instead of it
             line["mixed_precision"]["vs_oracle"] = {"error": str(e)}
+        stage("oracle mixed check done")
         b = batched_form()
+        stage(f"batched {b['value']:.4g} body-steps/s")
         b["ratio_vs_headline_f64"] = b["value"] / value
         line["batched"] = b
         s = served_form()
         s["ratio_vs_headline_f64"] = s["body_steps_per_s"] / value
         s["note"] = "offered load below capacity: body-steps/s is the arrival rate's"
         line["served"] = s
+        stage("served at 2/s done")
         s = served_form(requests=24, rate=16.0)
         s["ratio_vs_headline_f64"] = s["body_steps_per_s"] / value
         s["note"] = "arrivals outpace service: body-steps/s is the device's on this mix"
         line["served_saturated"] = s
+        stage("served at 16/s done")
         line["plain_gravity_explicit"] = plain_gravity_explicit()
+        stage("plain gravity explicit done")
         if not args.no_cpu_baseline:
             try:
                 line["cpu_baseline"] = cpu_baseline()

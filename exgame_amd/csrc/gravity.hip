@@ -27,9 +27,11 @@
 // replaces the reference's sqrt and three divisions; the result is within 2 ulp of
 // the correctly rounded form.
 // f32 pairs (mixed precision): positions are staged as double-single hi + lo pairs of
-// f32 (see f32ds_block), a pair costs one v_rsq_f32 and about fifteen f32 operations,
+// f32 (see f32ds_span), a pair costs one v_rsq_f32 and about fifteen f32 operations,
 // and the f32 sums of groups of 16 sources are added into f64 accumulators; the
-// reductions, scaling and moment are f64.
+// reductions, scaling and moment are f64. Pairs closer than three core sizes of the
+// target (the near field) are evaluated in f64 from the double-single records, and
+// pairs within a tile's own 64 slots in f64 by the reduce kernel.
 #include <hip/hip_runtime.h>
 #include "island.h"
 
@@ -171,20 +173,25 @@ __device__ inline void split_ds(double x, float& hi, float& lo) {
     lo = (float)(x - (double)hi);
 }
 
-// The f32 source records of bodies [0, n): x_hi y_hi z_hi G·m | x_lo y_lo z_lo 1/s.
+// The f32 source records of bodies [0, n): stage[2j] = x_hi y_hi z_hi G·m_hi,
+// stage[2j+1] = x_lo y_lo z_lo (1/s)_hi, and after the 2n records of the pair loop,
+// stage[2n + j] = G·m_lo (1/s)_lo 0 0, read only for near pairs.
 __global__ __launch_bounds__(256) void gravity_stage_f32_kernel(const double* __restrict__ ch, int n,
                                                                 float4* __restrict__ stage) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const double* c = ch + (size_t)j * 8;
-    float4 h, l;
+    float4 h, l, e;
     split_ds(c[0], h.x, l.x);
     split_ds(c[1], h.y, l.y);
     split_ds(c[2], h.z, l.z);
-    h.w = (float)c[6];
-    l.w = (float)(1.0 / c[7]);
+    split_ds(c[6], h.w, e.x);
+    split_ds(1.0 / c[7], l.w, e.y);
+    e.z = 0.f;
+    e.w = 0.f;
     stage[2 * (size_t)j] = h;
     stage[2 * (size_t)j + 1] = l;
+    stage[2 * (size_t)n + j] = e;
 }
 
 // Sources are read from the staged records with scalar loads (every lane of a wave
@@ -192,26 +199,74 @@ __global__ __launch_bounds__(256) void gravity_stage_f32_kernel(const double* __
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(4))) f4v* ConstF4;
 
+// The near field. A pair closer than three core sizes of its target (d² < 9 s_i², on
+// the f32 distance) is evaluated in f64 from the double-single records (positions,
+// G·m and 1/s to 2^-48) instead of in f32: the strongest pulls carry the largest f32
+// rounding. The pairs among a tile's own 64 slots are all left to the reduce kernel, in
+// f64: an island's bodies hold consecutive slots, and the f32 forces of its bodies on
+// each other would not cancel in the island's net force. Elsewhere near pairs are rare
+// (one in twenty thousand in the 4,096-body chains world, where they and the island
+// pairs hold three quarters of the f32 error of the islands' net forces), so the pair
+// loop stays branch-free: it skips them and notes, per lane, the least d² − 9 s_i² of
+// the group, and a group where some lane went below zero is passed over again for its
+// near pairs.
+constexpr float F32_NEAR_CORES2 = 9.f;
+
+struct F32Target {
+    float4 mh, ml;              // the target's records: x y z hi | G·m ; x y z lo | 1/s
+    float near2;                // 9 s², 0 for a lane without a target
+    double x, y, z, sinv;       // position and 1/s, hi + lo
+};
+
+// The f32 difference and squared distance of a pair, written out with explicit fused
+// multiply-adds so that the pair loop and the near-field pass compute the same bits.
+__device__ __forceinline__ float f32ds_d2(const f4v& h, const f4v& l, const F32Target& t, float& dx, float& dy,
+                                          float& dz) {
+    dx = (h.x - t.mh.x) + (l.x - t.ml.x);
+    dy = (h.y - t.mh.y) + (l.y - t.ml.y);
+    dz = (h.z - t.mh.z) + (l.z - t.ml.z);
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// The f64 pair (source jj on target i) from the double-single records.
 template <int MODE>
-__device__ __forceinline__ void f32ds_span(ConstF4 src, int j0, int cnt, int i, const float4& mh, const float4& ml,
-                                           float minv, double f[3]) {
+__device__ __forceinline__ void f64_near_pair(int jj, int i, const f4v& h, const f4v& l, const f4v& e,
+                                              const F32Target& t, double f[3]) {
+    const double dx = ((double)h.x + (double)l.x) - t.x;
+    const double dy = ((double)h.y + (double)l.y) - t.y;
+    const double dz = ((double)h.z + (double)l.z) - t.z;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    double inv_s = t.sinv;
+    if (MODE == 1) inv_s = (double)l.w + (double)e.y;
+    const double inv_r = fmin(rsqrt_refined(d2), inv_s);
+    const double w = d2 < 1e-18 ? 0.0 : ((double)h.w + (double)e.x) * (inv_r * inv_r * inv_r);
+    f[0] = fma(w, dx, f[0]);
+    f[1] = fma(w, dy, f[1]);
+    f[2] = fma(w, dz, f[2]);
+}
+
+// MODE 0: every source below every target (the higher slot, whose core size counts,
+// is the target); MODE 1: every source above.
+template <int MODE>
+__device__ __forceinline__ void f32ds_span(ConstF4 src, ConstF4 src_e, int j0, int cnt, int i, const F32Target& t,
+                                           double f[3]) {
+    static_assert(MODE == 0 || MODE == 1, "the tile's own slots are summed by the reduce kernel");
     int q = 0;
     while (q < cnt) {
         const int qe = min(cnt, q + F32_GROUP);
         float a[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+        // least d² − 9 s_i² of the group, in a vector register (a running lane mask
+        // would take scalar registers the batched source loads need)
+        float near_margin = 1.f;
         auto pair = [&](int k, float* acc) {
             const int jj = j0 + k;
             const f4v h = src[2 * jj], l = src[2 * jj + 1];
-            const float dx = (h.x - mh.x) + (l.x - ml.x);
-            const float dy = (h.y - mh.y) + (l.y - ml.y);
-            const float dz = (h.z - mh.z) + (l.z - ml.z);
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            float inv_s = minv;
-            if (MODE == 1) inv_s = l.w;
-            if (MODE == 2) inv_s = jj > i ? l.w : minv;
-            const float inv_r = fminf(rsqrt_refined_f32(d2), inv_s);
+            float dx, dy, dz;
+            const float d2 = f32ds_d2(h, l, t, dx, dy, dz);
+            const float inv_r = fminf(rsqrt_refined_f32(d2), MODE == 1 ? l.w : t.ml.w);
             float w = h.w * (inv_r * inv_r * inv_r);
-            w = ((MODE == 2 && jj == i) || d2 < 1e-18f) ? 0.f : w;
+            w = d2 < t.near2 ? 0.f : w;          // near (and coincident) pairs: not here
+            near_margin = fminf(near_margin, d2 - t.near2);
             acc[0] = fmaf(w, dx, acc[0]);
             acc[1] = fmaf(w, dy, acc[1]);
             acc[2] = fmaf(w, dz, acc[2]);
@@ -224,6 +279,32 @@ __device__ __forceinline__ void f32ds_span(ConstF4 src, int j0, int cnt, int i, 
         }
         if (k < qe) pair(k, a[0]);
         for (int c = 0; c < 3; ++c) f[c] += (double)a[0][c] + (double)a[1][c];
+        if (near_margin < 0.f) {   // rare: the group's near pairs in f64
+            // four sources at a time, loaded and tested before any branch so that a
+            // batch's loads are in flight together
+            for (int kk = q; kk < qe; kk += 4) {
+                f4v h[4], l[4];
+                bool nearp[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j0 + min(kk + u, qe - 1);
+                    h[u] = src[2 * jj];
+                    l[u] = src[2 * jj + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    float dx, dy, dz;
+                    const float d2 = f32ds_d2(h[u], l[u], t, dx, dy, dz);
+                    nearp[u] = kk + u < qe && d2 < t.near2;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (nearp[u]) {
+                        const f4v e = src_e[j0 + kk + u];
+                        f64_near_pair<MODE>(j0 + kk + u, i, h[u], l[u], e, t, f);
+                    }
+            }
+        }
         q = qe;
     }
 }
@@ -231,48 +312,49 @@ __device__ __forceinline__ void f32ds_span(ConstF4 src, int j0, int cnt, int i, 
 __global__ __launch_bounds__(64 * GRAV_WAVES) void gravity_partial_f32c_kernel(const double* __restrict__ ch,
                                                                                const GravEntry* __restrict__ entries,
                                                                                const float4* __restrict__ stage,
+                                                                               int n_stage,
                                                                                double* __restrict__ partial) {
     const GravEntry en = entries[blockIdx.x];
     // the wave's index is wave-uniform: its slice bounds and loop counts live in scalar registers
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int i = en.tgt0 + lane;
     const bool live = lane < en.ntgt;
-    float4 mh = make_float4(0.f, 0.f, 0.f, 0.f), ml = make_float4(0.f, 0.f, 0.f, 0.f);
-    float minv = 1.f;
+    F32Target t;
+    t.mh = make_float4(0.f, 0.f, 0.f, 0.f);
+    t.ml = make_float4(0.f, 0.f, 0.f, 1.f);
+    t.near2 = 0.f;
+    t.x = t.y = t.z = 0.0;
+    t.sinv = 1.0;
     if (live) {
-        const float4 h = stage[2 * (size_t)i], l = stage[2 * (size_t)i + 1];
-        mh = h;
-        ml = l;
-        minv = l.w;
+        const float4 h = stage[2 * (size_t)i], l = stage[2 * (size_t)i + 1], e = stage[2 * (size_t)n_stage + i];
+        t.mh = h;
+        t.ml = l;
+        t.x = (double)h.x + (double)l.x;
+        t.y = (double)h.y + (double)l.y;
+        t.z = (double)h.z + (double)l.z;
+        t.sinv = (double)l.w + (double)e.y;
+        const float sz = 1.f / l.w;
+        t.near2 = F32_NEAR_CORES2 * sz * sz;
     }
     const int len = en.src1 - en.src0, per = (len + GRAV_WAVES - 1) / GRAV_WAVES;
     const int s0 = en.src0 + min(len, w * per), s1 = en.src0 + min(len, (w + 1) * per);
     const int t_lo = en.tgt0, t_hi = en.tgt0 + en.ntgt - 1;
-    ConstF4 src = (ConstF4)stage;
+    ConstF4 src = (ConstF4)stage, src_e = src + 2 * (size_t)n_stage;
     double f[3] = {0.0, 0.0, 0.0};
+    // sources in blocks of 64, each clipped around the tile's own slots [t_lo, t_hi]
+    // (those pairs are the reduce kernel's)
     for (int base = s0; base < s1; base += 64) {
-        const int cnt = min(64, s1 - base);
-        if (base + cnt - 1 < t_lo) f32ds_span<0>(src, base, cnt, i, mh, ml, minv, f);
-        else if (base > t_hi) f32ds_span<1>(src, base, cnt, i, mh, ml, minv, f);
-        else f32ds_span<2>(src, base, cnt, i, mh, ml, minv, f);
+        const int end = min(base + 64, s1);
+        const int below = min(end, t_lo);
+        if (base < below) f32ds_span<0>(src, src_e, base, below - base, i, t, f);
+        const int above = max(base, t_hi + 1);
+        if (above < end) f32ds_span<1>(src, src_e, above, end - above, i, t, f);
     }
     write_partial(f[0], f[1], f[2], live, partial);
 }
 
-__global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double* __restrict__ ch,
-                                                                   const GravTile* __restrict__ tiles,
-                                                                   const double* __restrict__ partial,
-                                                                   double* __restrict__ ext) {
-    const GravTile tl = tiles[blockIdx.x];
-    const int lane = threadIdx.x;
-    if (lane >= tl.ntgt) return;
-    const int i = tl.tgt0 + lane;
-    double* o = ext + (size_t)i * 6;
-    if (tl.ne == 0) {  // a world without gravity
-        for (int k = 0; k < 6; ++k) o[k] = 0.0;
-        return;
-    }
-    double f[3] = {0.0, 0.0, 0.0};
+// A tile's entries added in ascending chunk order (one target per lane).
+__device__ inline void sum_partials(const GravTile& tl, const double* __restrict__ partial, int lane, double f[3]) {
     int c = 0;
     for (; c + 4 <= tl.ne; c += 4) {  // loads issued four chunks at a time, added in order
         double v[4][3];
@@ -288,6 +370,12 @@ __global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double*
     }
     for (; c < tl.ne; ++c)
         for (int k = 0; k < 3; ++k) f[k] += partial[(size_t)(tl.e0 + c) * GRAV_PARTIAL_PER_ENTRY + k * GRAV_TILE + lane];
+}
+
+// Scale by the target's charge / G and write the wrench with the moment local × F.
+__device__ inline void write_wrench(const double* __restrict__ ch, const GravTile& tl, int i, double f[3],
+                                    double* __restrict__ ext) {
+    double* o = ext + (size_t)i * 6;
     const double a = ch[(size_t)i * 8 + 6] * tl.g_inv;
     for (int k = 0; k < 3; ++k) f[k] *= a;
     const double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
@@ -297,6 +385,79 @@ __global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double*
     o[3] = ly * f[2] - lz * f[1];
     o[4] = lz * f[0] - lx * f[2];
     o[5] = lx * f[1] - ly * f[0];
+}
+
+__global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double* __restrict__ ch,
+                                                                   const GravTile* __restrict__ tiles,
+                                                                   const double* __restrict__ partial,
+                                                                   double* __restrict__ ext) {
+    const GravTile tl = tiles[blockIdx.x];
+    const int lane = threadIdx.x;
+    if (lane >= tl.ntgt) return;
+    const int i = tl.tgt0 + lane;
+    if (tl.ne == 0) {  // a world without gravity
+        for (int k = 0; k < 6; ++k) ext[(size_t)i * 6 + k] = 0.0;
+        return;
+    }
+    double f[3] = {0.0, 0.0, 0.0};
+    sum_partials(tl, partial, lane, f);
+    write_wrench(ch, tl, i, f, ext);
+}
+
+// The f32 pairs' reduce: the entries in ascending chunk order, then the pairs among the
+// tile's own slots (which the f32 pair kernel leaves out) in f64 with the f64 kernel's
+// pair arithmetic. Wave 0 sums the entries while waves 1-3 take the tile's sources
+// [0, 21), [21, 42), [42, 64) from LDS; their sums are added in wave order.
+__global__ __launch_bounds__(GRAV_TILE * GRAV_WAVES) void gravity_reduce_f32c_kernel(
+    const double* __restrict__ ch, const GravTile* __restrict__ tiles, const double* __restrict__ partial,
+    double* __restrict__ ext) {
+    static_assert(GRAV_WAVES == 4 && GRAV_TILE == 64, "three waves of tile sources");
+    __shared__ double tc[GRAV_TILE][5];                  // x y z G·m 1/s of the tile's bodies
+    __shared__ double dsum[GRAV_WAVES - 1][3][GRAV_TILE];
+    const GravTile tl = tiles[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = tl.tgt0 + lane;
+    if (tl.ne == 0) {  // a world without gravity
+        if (w == 0 && lane < tl.ntgt)
+            for (int k = 0; k < 6; ++k) ext[(size_t)i * 6 + k] = 0.0;
+        return;
+    }
+    if (w == 1 && lane < tl.ntgt) {
+        const double* c = ch + (size_t)i * 8;
+        tc[lane][0] = c[0];
+        tc[lane][1] = c[1];
+        tc[lane][2] = c[2];
+        tc[lane][3] = c[6];
+        tc[lane][4] = 1.0 / c[7];
+    }
+    double f[3] = {0.0, 0.0, 0.0};
+    if (w == 0 && lane < tl.ntgt) sum_partials(tl, partial, lane, f);
+    __syncthreads();
+    if (w > 0) {
+        double g[3] = {0.0, 0.0, 0.0};
+        if (lane < tl.ntgt) {
+            const double xi = tc[lane][0], yi = tc[lane][1], zi = tc[lane][2], sinv_i = tc[lane][4];
+            const int ja = (w - 1) * 21, jb = w == 3 ? tl.ntgt : min(w * 21, tl.ntgt);
+#pragma unroll 4
+            for (int j = ja; j < jb; ++j) {
+                const double dx = tc[j][0] - xi, dy = tc[j][1] - yi, dz = tc[j][2] - zi;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                const double inv_s = j > lane ? tc[j][4] : sinv_i;
+                const double inv_r = fmin(rsqrt_refined(d2), inv_s);
+                double wt = tc[j][3] * (inv_r * inv_r * inv_r);
+                wt = (j == lane || d2 < 1e-18) ? 0.0 : wt;
+                g[0] = fma(wt, dx, g[0]);
+                g[1] = fma(wt, dy, g[1]);
+                g[2] = fma(wt, dz, g[2]);
+            }
+        }
+        for (int k = 0; k < 3; ++k) dsum[w - 1][k][lane] = g[k];
+    }
+    __syncthreads();
+    if (w != 0 || lane >= tl.ntgt) return;
+    for (int v = 0; v < GRAV_WAVES - 1; ++v)
+        for (int k = 0; k < 3; ++k) f[k] += dsum[v][k][lane];
+    write_wrench(ch, tl, i, f, ext);
 }
 
 // Bodies of worlds without gravity get a zero wrench (a fill kernel, so a step
@@ -369,12 +530,16 @@ void launch_gravity(const double* charges, const GravEntry* entries, int n_entri
         launch_k(gravity_stage_f32_kernel, dim3((n_bodies + 255) / 256), dim3(256), stream, start, nullptr, charges,
                  n_bodies, (float4*)stage);
         launch_k(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, nullptr, nullptr, charges,
-                 entries, (const float4*)stage, partial);
+                 entries, (const float4*)stage, n_bodies, partial);
     } else
         launch_k(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
                  entries, partial);
-    launch_k(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), stream, nullptr, stop, charges, tiles,
-             (const double*)partial, ext);
+    if (precision == 1)
+        launch_k(gravity_reduce_f32c_kernel, dim3(n_tiles), dim3(GRAV_TILE * GRAV_WAVES), stream, nullptr, stop, charges,
+                 tiles, (const double*)partial, ext);
+    else
+        launch_k(gravity_reduce_kernel, dim3(n_tiles), dim3(GRAV_TILE), stream, nullptr, stop, charges, tiles,
+                 (const double*)partial, ext);
 }
 
 void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream, hipEvent_t start, hipEvent_t stop) {

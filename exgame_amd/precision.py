@@ -14,7 +14,10 @@ rollout (positions and momenta separately). The FP64 rollout itself is the devic
 path that the tests pin against the oracle at smaller sizes (the oracle cannot step
 65,536 bodies for 200 steps in reasonable time).
 
-    python -m exgame_amd.precision [--gravity-n 262144] [--chains 8192] [--steps 200] [--radius 50]
+    python -m exgame_amd.precision [--gravity-n 262144] [--chains 8192] [--steps 200] [--radius R]
+
+The chain origins fill a ball of radius 50·(chains/512)^(1/3) unless --radius is given:
+the 4,096-body workload's density at any size (126 at 8,192 chains).
 
 Prints one JSON line.
 """
@@ -130,14 +133,16 @@ def main() -> int:
     ap.add_argument("--gravity-n", type=int, default=262144)
     ap.add_argument("--chains", type=int, default=8192)
     ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--radius", type=float, default=50.0, help="radius of the ball of chain origins")
+    ap.add_argument("--radius", type=float, default=None,
+                    help="radius of the ball of chain origins (default: the 4,096-body workload's density)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--skip-gravity", action="store_true")
     args = ap.parse_args()
     line = {"form": "precision"}
     if not args.skip_gravity:
         line["gravity"] = gravity_form(args.gravity_n, args.seed)
-    line["jointed"] = jointed_form(args.chains, args.steps, args.radius)
+    radius = args.radius if args.radius is not None else 50.0 * (args.chains / 512.0) ** (1.0 / 3.0)
+    line["jointed"] = jointed_form(args.chains, args.steps, radius)
     print(json.dumps(line), flush=True)
     return 0
 
