@@ -123,7 +123,7 @@ __device__ inline void mark_fatal(const IslandArgs& a, int isl) {
 // mapping (EXG_JOINT_MAPPING_FLAT): island_class never returns it; a world or an
 // arena world that asks for that mapping routes its m8j14 islands there.
 enum {
-    ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J8 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASS_LARGE = 4,
+    ISLAND_CLASS_M1 = 0, ISLAND_CLASS_M4J10 = 1, ISLAND_CLASS_M8J14 = 2, ISLAND_CLASS_M8J16 = 3, ISLAND_CLASS_LARGE = 4,
     ISLAND_CLASS_M8J14_FLAT = 5, ISLAND_CLASSES = 6
 };
 int island_class(int m, int joints);   // -1 when no variant holds the island
@@ -146,7 +146,7 @@ inline void launch_k(F kernel, dim3 grid, dim3 block, hipStream_t stream, hipEve
 void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEvent_t start = nullptr,
                          hipEvent_t stop = nullptr);
 // Every island class of one step, run side by side: the classes with islands are
-// launched heaviest first (large, m8j14, m8j14 flat, m8j16, m4j8, singles), the first on `main`
+// launched heaviest first (large, m8j14, m8j14 flat, m8j16, m4j10, singles), the first on `main`
 // and each further one on its own side stream after `fork` (recorded on main), and
 // `main` then waits for `join[c]` of each. Classes hold disjoint islands, so their
 // order does not change any result. `class_ids[c]` lists class c's islands. With a

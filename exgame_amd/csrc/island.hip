@@ -1173,13 +1173,14 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
 // Kernel variants by island class (the host routes each island to the smallest
 // that holds it; grids run over that class's island ids):
 //   m1    single bodies without joints: one thread per island (singles.hip)
-//   m4j8  up to 4 bodies / 8 joints: 128 threads
+//   m4j10 up to 4 bodies / 10 joints (a 4-body rig: springs and torsional dampers
+//         around the ring and across one diagonal): 128 threads
 //   m8j14 up to 8 bodies / 14 joints: the shared state fits two workgroups per CU
 //         (two islands in flight, 2 waves per SIMD, ≤ 256 registers per lane)
 //   m8j16 up to 8 bodies / 16 joints: one workgroup holds the CU
-__global__ __launch_bounds__(128) void island_step_kernel_m4j8(IslandArgs a) {
-    __shared__ IslandShared<4, 8, 128, false> s;
-    island_step<4, 8, 128, false>(a, s);
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m4j10(IslandArgs a) {
+    __shared__ IslandShared<4, 10, 128, false> s;
+    island_step<4, 10, 128, false>(a, s);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14(IslandArgs a) {
     __shared__ IslandShared<8, 14, 256, false> s;
@@ -1198,9 +1199,9 @@ __global__ __launch_bounds__(256) void island_step_kernel_m8j16(IslandArgs a) {
 }
 // Mixed precision: FP32 Jacobian lanes and compensated FP32 assembly (f64 residual,
 // Newton–Schulz and state update).
-__global__ __launch_bounds__(128) void island_step_kernel_m4j8_jf32(IslandArgs a) {
-    __shared__ IslandShared<4, 8, 128, true> s;
-    island_step<4, 8, 128, true>(a, s);
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m4j10_jf32(IslandArgs a) {
+    __shared__ IslandShared<4, 10, 128, true> s;
+    island_step<4, 10, 128, true>(a, s);
 }
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void island_step_kernel_m8j14_jf32(IslandArgs a) {
     __shared__ IslandShared<8, 14, 256, true> s;
@@ -1217,7 +1218,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void i
 
 int island_class(int m, int j) {
     if (m == 1 && j == 0) return ISLAND_CLASS_M1;
-    if (m <= 4 && j <= 8) return ISLAND_CLASS_M4J8;
+    if (m <= 4 && j <= 10) return ISLAND_CLASS_M4J10;
     if (m <= 8 && j <= 14) return ISLAND_CLASS_M8J14;
     if (m <= 8 && j <= 16) return ISLAND_CLASS_M8J16;
     if (m <= LARGE_MAXM && j <= LARGE_MAXJ) return ISLAND_CLASS_LARGE;
@@ -1226,9 +1227,9 @@ int island_class(int m, int j) {
 
 void island_class_shape(int cls, int& waves, int& per_wave) {
     switch (cls) {
-        case ISLAND_CLASS_M4J8:
-            waves = IslandShared<4, 8, 128, false>::WAVES;
-            per_wave = IslandShared<4, 8, 128, false>::PER_WAVE;
+        case ISLAND_CLASS_M4J10:
+            waves = IslandShared<4, 10, 128, false>::WAVES;
+            per_wave = IslandShared<4, 10, 128, false>::PER_WAVE;
             break;
         case ISLAND_CLASS_M8J14:
         case ISLAND_CLASS_M8J14_FLAT:
@@ -1265,7 +1266,7 @@ void ClassStreams::destroy() {
 
 hipError_t launch_classes(IslandArgs a, const ClassLaunch& c, hipEvent_t start, hipEvent_t stop) {
     static const int order[ISLAND_CLASSES] = {ISLAND_CLASS_LARGE, ISLAND_CLASS_M8J14, ISLAND_CLASS_M8J14_FLAT,
-                                              ISLAND_CLASS_M8J16, ISLAND_CLASS_M4J8, ISLAND_CLASS_M1};
+                                              ISLAND_CLASS_M8J16, ISLAND_CLASS_M4J10, ISLAND_CLASS_M1};
     int live[ISLAND_CLASSES], nl = 0;
     for (int k = 0; k < ISLAND_CLASSES; ++k)
         if (c.n[order[k]] > 0) live[nl++] = order[k];
@@ -1300,9 +1301,9 @@ void launch_island_class(const IslandArgs& a, int cls, hipStream_t stream, hipEv
         case ISLAND_CLASS_M1:
             launch_singles(a, stream, start, stop);
             break;
-        case ISLAND_CLASS_M4J8:
-            if (a.jac_f32) launch_k(island_step_kernel_m4j8_jf32, grid, dim3(128), stream, start, stop, a);
-            else launch_k(island_step_kernel_m4j8, grid, dim3(128), stream, start, stop, a);
+        case ISLAND_CLASS_M4J10:
+            if (a.jac_f32) launch_k(island_step_kernel_m4j10_jf32, grid, dim3(128), stream, start, stop, a);
+            else launch_k(island_step_kernel_m4j10, grid, dim3(128), stream, start, stop, a);
             break;
         case ISLAND_CLASS_M8J14:
             if (a.jac_f32) launch_k(island_step_kernel_m8j14_jf32, grid, dim3(256), stream, start, stop, a);

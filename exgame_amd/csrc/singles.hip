@@ -13,9 +13,10 @@
 //   waves per CU), and, inside a Newton–Schulz iteration, X and the product T in
 //   registers. Each lane runs the reference's sequential arithmetic on its own body,
 //   so a wave holds 64 islands and no lane waits on another for data. A body whose
-//   first span does not converge (its step subdivides) is given up before anything of
-//   it is written and its island goes on a requeue list: a wave lasts as long as its
-//   slowest island, and one such body would hold 63 others.
+//   first span does not converge (its step subdivides), or needs more than BAIL_WORK
+//   residual evaluations, is given up before anything of it is written and its island
+//   goes on a requeue list: a wave lasts as long as its slowest island, and one such
+//   body would hold 63 others.
 // * singles_requeue_kernel, eight lanes per island over the requeue list: lane r (r <
 //   6; lanes 6 and 7 mirror row 5) holds row r of A, X, T and the candidate-0 mass
 //   block in registers and forms that row of every product, the other rows coming from
@@ -48,6 +49,11 @@ constexpr int SB = 64;           // one-lane kernel: threads per workgroup, one 
 constexpr int SG = 8;            // requeue kernel: lanes per island
 constexpr int SB8 = 256;         // requeue kernel: threads per workgroup (32 islands)
 constexpr int MAX_IT = 32;
+// One-lane kernel: residual evaluations (Newton iterations and line-search lanes) a
+// first span may take before its island is handed to the requeue kernel. A free body
+// converges in about four Newton iterations (eight evaluations); bodies in close
+// encounters take many more, and one such lane would hold its 63 wave-mates.
+constexpr int BAIL_WORK = 16;
 constexpr int DEPTH = 8;
 constexpr int THRESH = 8;
 constexpr int PER_HALVING = 4;
@@ -259,6 +265,8 @@ struct Free {
     double an;
     double dt, warp, half, span;
     bool seeded;
+    int bail_work = 0;      // > 0: give the span up after this many residual evaluations
+    bool bailed = false;
     int st_spans = 0, st_newton = 0, st_ns = 0, st_reseed = 0, st_reject = 0, st_uncommit = 0, st_nonfinite = 0;
 
     // The residual of a candidate velocity (pre + gather + body_post + wnorm for one
@@ -368,8 +376,13 @@ struct Free {
         assembled = false;
         bool has_carried = false;
         double carried = 0.0;
+        int work = 0;
 #pragma clang loop unroll(disable)
         for (int iter = 0; iter < MAX_IT; ++iter) {
+            if (bail_work > 0 && work >= bail_work) {
+                bailed = true;
+                break;
+            }
             double rnorm;
             st_newton++;
             if (has_carried) {
@@ -377,6 +390,7 @@ struct Free {
                 rnorm = carried;
             } else {
                 rnorm = evaluate_block(vmid, rhs);
+                work++;
             }
             if (rnorm < best) {
                 best = rnorm;
@@ -422,6 +436,7 @@ struct Free {
                 // lane 0's block is the iterate's from here on (kept for signal_log2 when
                 // every lane fails); a later lane's block is formed again on adoption
                 const double w = l == 0 ? evaluate_block(v, r_l) : evaluate_ratio(v, r_l);
+                work++;
                 if (w < rnorm) {
                     accepted = l;
                     tnorm = w;
@@ -476,8 +491,10 @@ struct Free {
 #pragma unroll
         for (int e = 0; e < 36; ++e) s_inv[e][ln] = a.xacc[xoff + e];
     }
-    // With `bail`, a first span that does not converge ends the solve before anything
-    // is committed or written (returns false: the island goes to the requeue kernel).
+    // With `bail`, a first span that does not converge, or that takes more than
+    // BAIL_WORK residual evaluations, ends the solve before anything is committed or
+    // written (returns false: the island goes to the requeue kernel, which solves the
+    // step from its start with the same arithmetic).
     __device__ __forceinline__ bool solve_step(double tol2, int ns_warm, int ns_seed, bool bail) {  // newton.rs:543-683
         const long long xoff = a.xacc_off[isl];
         bool accepted_seeded = seeded;
@@ -496,7 +513,9 @@ struct Free {
             world_momentum(pose, mom, snap);
             bool converged, assembled;
             double best;
+            bail_work = bail && depth == 0 ? BAIL_WORK : 0;
             newton_solve(tol2, ns_warm, ns_seed, converged, best, assembled);
+            if (bailed) return false;
             const bool finite = best < 1000000000.0;
             const bool last = depth == DEPTH;
             if (!converged && !last) {
