@@ -25,9 +25,12 @@
 //   sooner than one lane does. Its last workgroup clears the list's counters for the
 //   next step (no memset: the step stays capturable in a graph).
 //
-// Both solve the whole step from its start with the same arithmetic in the same order
-// (bit-identical results), so which kernel finishes a body changes nothing. Without a
-// requeue list (IslandArgs::requeue null) the one-lane kernel solves every step itself.
+// Both solve the whole step from its start with the same arithmetic in the same order;
+// the compiler fuses multiply-adds differently in the two kernels, so they agree to
+// rounding, not bit for bit. Which kernel finishes a body is decided by that body's own
+// state alone, so a world's results do not depend on what else shares the launch.
+// Without a requeue list (IslandArgs::requeue null) the one-lane kernel solves every
+// step itself.
 //
 // Line-search lanes are evaluated in order with early exit: the first lane whose
 // residual drops below the current one is accepted, which is the lane the reference's

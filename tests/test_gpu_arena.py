@@ -189,3 +189,29 @@ def test_autotune_changes_only_later_admissions():
     b3 = alone3.admit(w3)
     alone3.step(3, 1e-2, 1.0)
     assert same_state(mix.state(a3), alone3.state(b3))
+
+
+def test_subdividing_free_bodies_agree_as_world_alone_and_mixed():
+    # free bodies in close encounters (a dense ball, dt = 1e-2): bodies whose first span
+    # goes beyond the one-lane kernel's budget are finished by the eight-lane kernel
+    # from the requeue list, decided by each body's own state alone; a single world,
+    # the same world alone in an arena and beside another world give the same bits
+    scene = scenes.plain_gravity(n=600, seed=4)
+    dev = DeviceWorld(scene, tol2=1e-20)
+    alone, mixed = big_arena(tol2=1e-20), big_arena(tol2=1e-20)
+    wa = alone.admit(scene)
+    other = mixed.admit(scenes.mixed_world(800, seed=9))
+    wm = mixed.admit(scene)
+    for n in (3, 4, 2, 5):
+        dev.step(n, 1e-2, 1.0)
+        alone.step(n, 1e-2, 1.0)
+        mixed.step(n, 1e-2, 1.0)
+        d, a, b = dev.state(), alone.state(wa), mixed.state(wm)
+        for k in ("pose", "momentum", "origin"):
+            assert np.array_equal(d[k], a[k]), (n, k)
+            assert np.array_equal(a[k], b[k]), (n, k)
+    st = dev.stats()
+    assert int(st[:, 0].max()) > 17, "the scene should subdivide some steps"
+    mixed.retire(other)
+    alone.sync()
+    mixed.sync()
