@@ -1111,34 +1111,44 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     __syncthreads();
 
     // ── recompute_centroid (island.rs:206-244) and publish the charges ──
+    // One thread per body forms its position (into snap, whose span is over); thread 0
+    // adds them in body order and forms the shift; one thread per body composes its pose
+    // with the shift and checks its state. (best holds the shift: the solve is over.)
+    double* shift_sh = &s.best[0][0];
+    if (threadIdx.x < m) body_position(s.pose[threadIdx.x], s.snap[threadIdx.x]);
+    if (threadIdx.x == 0) {
+        s.ibcast[1] = sv.seeded ? 1 : 0;
+        s.ibcast[2] = 0;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         double total_m = 0.0, wx[3] = {0.0, 0.0, 0.0};
         for (int i = 0; i < m; ++i) {
-            double p[3];
-            body_position(s.pose[i], p);
             total_m += s.massv[i];
-            for (int k = 0; k < 3; ++k) wx[k] += p[k] * s.massv[i];
+            for (int k = 0; k < 3; ++k) wx[k] += s.snap[i][k] * s.massv[i];
         }
-        if (!effective_zero(total_m)) {
+        const bool shifted = !effective_zero(total_m);
+        if (shifted) {
             double inv = 1.0 / total_m;
             double sh[6] = {-(wx[0] * inv), -(wx[1] * inv), -(wx[2] * inv), 0.0, 0.0, 0.0};
-            double shift[8];
-            twist_exp(sh, 1.0, shift);
-            for (int i = 0; i < m; ++i) {
-                double np[8];
-                compose(s.pose[i], shift, np);
-                for (int k = 0; k < 8; ++k) s.pose[i][k] = np[k];
-            }
+            twist_exp(sh, 1.0, shift_sh);
             for (int k = 0; k < 3; ++k) a.origin[isl * 3 + k] += wx[k] * inv;
         }
+        s.ibcast[3] = shifted ? 1 : 0;
         for (int k = 0; k < 3; ++k) s.bcast[2 + k] = a.origin[isl * 3 + k];
-        s.ibcast[1] = sv.seeded ? 1 : 0;
-        bool bad = false;
-        for (int i = 0; i < m; ++i) {
-            for (int k = 0; k < 8; ++k) bad |= !isfinite(s.pose[i][k]);
-            for (int k = 0; k < 6; ++k) bad |= !isfinite(s.mom[i][k]);
+    }
+    __syncthreads();
+    if (threadIdx.x < m) {
+        const int i = threadIdx.x;
+        if (s.ibcast[3]) {
+            double np[8];
+            compose(s.pose[i], shift_sh, np);
+            for (int k = 0; k < 8; ++k) s.pose[i][k] = np[k];
         }
-        s.ibcast[2] = bad ? 1 : 0;
+        bool bad = false;
+        for (int k = 0; k < 8; ++k) bad |= !isfinite(s.pose[i][k]);
+        for (int k = 0; k < 6; ++k) bad |= !isfinite(s.mom[i][k]);
+        if (bad) s.ibcast[2] = 1;
     }
     __syncthreads();
     double org[3] = {s.bcast[2], s.bcast[3], s.bcast[4]};
