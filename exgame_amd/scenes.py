@@ -130,8 +130,11 @@ class Scene:
 
 
 def effective_size(mass: np.ndarray) -> np.ndarray:
-    """(m/4)^(1/3) — crates/newton/src/body.rs:70-77, with the host's libm pow."""
-    return np.array([float(np.power(m / 4.0, 1.0 / 3.0)) for m in np.asarray(mass, dtype=np.float64)])
+    """(m/4)^(1/3) — crates/newton/src/body.rs:70-77, with the host's libm pow. A
+    kinematic body (mass 0) gets 1: its pairs carry no mass, so its size never enters a
+    force."""
+    return np.array([float(np.power(m / 4.0, 1.0 / 3.0)) if m != 0.0 else 1.0
+                     for m in np.asarray(mass, dtype=np.float64)])
 
 
 def _assemble(positions, rotvecs, mass, tensors, vbody, island_sizes, joints, meta) -> Scene:
@@ -145,11 +148,14 @@ def _assemble(positions, rotvecs, mass, tensors, vbody, island_sizes, joints, me
     for k in range(I):
         s, e = island_start[k], island_start[k + 1]
         mm = mass[s:e]
-        com = (positions[s:e] * mm[:, None]).sum(0) / mm.sum()
+        # an island without dynamic mass keeps its first body's position as its origin
+        com = (positions[s:e] * mm[:, None]).sum(0) / mm.sum() if mm.sum() > 0.0 else positions[s].copy()
         origin[k] = com
         local[s:e] = positions[s:e] - com
     pose = pga.pose_from(local, rotvecs)
     mom = pga.inertia_apply(mass, tensors.reshape(B, 9), vbody)
+    kin = np.asarray(mass) == 0.0
+    mom[kin] = vbody[kin]   # a kinematic body's momentum slot holds its velocity (inertia.rs:135)
     if joints:
         jk, ja, jb, jp = zip(*joints)
         joint_kind = np.array(jk, dtype=np.int32)
@@ -247,6 +253,40 @@ def jointed_chains(n_chains: int = 512, m: int = 8, seed: int = 0, radius: float
     return _assemble(pos, rot, mass, tens, vb, np.full(n_chains, m, dtype=np.int64), joints,
                      {"workload": "jointed_chains", "n_chains": n_chains, "m": m, "seed": seed, "radius": radius,
                       "braced": braced})
+
+
+def anchored_chains(n_chains: int = 16, m: int = 8, seed: int = 0, radius: float = 20.0, k: float = 1e3,
+                    c: float = 10.0, c_tors: float = 5.0, eps: float = 1e-6, anchor_speed: float = 0.2,
+                    kinematic_first: bool = False) -> Scene:
+    """Chains of `m` bodies whose first body (along the chain) is kinematic (mass 0,
+    Inertia::Kinematic, crates/newton/src/inertia.rs:103): it is a reference that the
+    integrators never move, with a prescribed body-frame velocity of components
+    N(0, anchor_speed²) that its joints' dampers see. The other bodies and joints are
+    the jointed-chain ones. The anchor's slot is the island's last (the layout the
+    device kernels take: dynamic bodies first) or, with `kinematic_first`, its first."""
+    st = Stream(seed, 7)
+    pos, links = chain_bodies(st, n_chains, m, radius)
+    n = n_chains * m
+    mass, tens, rot, vb = rigid_draws(st, n)
+    anchor_v = anchor_speed * np.stack([st.normal(n_chains) for _ in range(6)], axis=-1)
+    # along-chain index q → slot: the anchor (q = 0) last, or first
+    slot = np.array([(m - 1 if q == 0 else q - 1) if not kinematic_first else q for q in range(m)])
+    P, Mv, T, R, V = (np.zeros_like(pos), np.zeros(n), np.zeros_like(tens), np.zeros_like(rot), np.zeros_like(vb))
+    joints = []
+    for ch in range(n_chains):
+        b0 = ch * m
+        for q in range(m):
+            src, dst = b0 + q, b0 + slot[q]
+            P[dst], R[dst] = pos[src], rot[src]
+            if q == 0:
+                Mv[dst], T[dst], V[dst] = 0.0, np.zeros((3, 3)), anchor_v[ch]
+            else:
+                Mv[dst], T[dst], V[dst] = mass[src], tens[src], vb[src]
+        for (kind, a, b, prm) in chain_joints(0, links[ch], k, c, c_tors, eps):
+            joints.append((kind, b0 + int(slot[a]), b0 + int(slot[b]), prm))
+    return _assemble(P, R, Mv, T, V, np.full(n_chains, m, dtype=np.int64), joints,
+                     {"workload": "anchored_chains", "n_chains": n_chains, "m": m, "seed": seed,
+                      "anchor_speed": anchor_speed, "kinematic_first": kinematic_first})
 
 
 def cloth_grids(shapes=((4, 4), (5, 9)), seed: int = 0, radius: float = 20.0, k: float = 1e3, c: float = 10.0,

@@ -258,6 +258,7 @@ void orc_ns_solve(int m, const double* a, const double* rhs6m, int iters, double
                   double* rho_out) {
     Island isl;
     isl.bodies.resize(m);
+    for (Body& b : isl.bodies) b.inertia.mass = 1.0;   // dynamic: one block row each
     cache_sync(isl);
     for (int b = 0; b < m * m; ++b)
         for (int r = 0; r < 6; ++r) for (int c = 0; c < 6; ++c) isl.A[b][r][c] = a[b * 36 + r * 6 + c];

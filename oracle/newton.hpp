@@ -52,12 +52,14 @@ constexpr int NO_SIGNAL = -100000;
 struct SpanEvent { int depth, converged, committed, signal; };
 
 struct Island {
-    std::vector<Body> bodies;   // every body is dynamic; index = row of the system
+    std::vector<Body> bodies;   // dynamic and kinematic bodies, in island order
     std::vector<Joint> joints;
     double origin[3] = {0.0, 0.0, 0.0};
 
-    // NewtonCache (cache.rs:200-320)
+    // NewtonCache (cache.rs:200-320): the unknowns are the dynamic bodies, in island
+    // order (`order`, newton.rs:806-830); block row p belongs to body order[p]
     int m = -1;
+    std::vector<int> order, pos;   // pos[body]: its block row, −1 for a kinematic body
     std::vector<Block> A, X, Xn, Xacc, R;
     std::vector<std::vector<int>> neighbours;
     std::vector<std::vector<BlockTerm>> block_terms;
@@ -66,20 +68,30 @@ struct Island {
     std::vector<SpanEvent> trace;   // the spans of the last step
 };
 
+// Per body (every body of the island): vmid, midpoint, solve_vel, total; per block
+// row (the unknowns): rhs, scale, mass.
 struct Lane {
     std::vector<M16> vmid, midpoint, solve_vel;
     std::vector<std::array<std::array<double, 6>, 2>> conn;
     std::vector<std::array<double, 6>> total, rhs, scale;
     std::vector<Block> mass;
-    void resize(int m, int nj) {
-        vmid.resize(m); midpoint.resize(m); solve_vel.resize(m); conn.resize(nj);
-        total.resize(m); rhs.resize(m); scale.resize(m); mass.assign(m, zero_block());
+    void resize(int nb, int m, int nj) {
+        vmid.resize(nb); midpoint.resize(nb); solve_vel.resize(nb); conn.resize(nj);
+        total.resize(nb); rhs.resize(m); scale.resize(m); mass.assign(m, zero_block());
     }
 };
 
 // cache.sync: dimension latch, sparsity and the assembly table (cache.rs:566-788)
 inline void cache_sync(Island& isl) {
-    int m = (int)isl.bodies.size();
+    const int nb = (int)isl.bodies.size();
+    isl.order.clear();
+    isl.pos.assign(nb, -1);
+    for (int i = 0; i < nb; ++i)
+        if (!is_kinematic(isl.bodies[i].inertia)) {
+            isl.pos[i] = (int)isl.order.size();
+            isl.order.push_back(i);
+        }
+    int m = (int)isl.order.size();
     if (m != isl.m) {
         isl.A.assign(m * m, zero_block());
         isl.X.assign(m * m, zero_block());
@@ -95,14 +107,17 @@ inline void cache_sync(Island& isl) {
     isl.block_terms.assign(m * m, {});
     for (int jn = 0; jn < (int)isl.joints.size(); ++jn) {
         const Joint& e = isl.joints[jn];
-        int ia = e.a, ib = e.b;
-        auto& na = isl.neighbours[ia];
-        if (std::find(na.begin(), na.end(), ib) == na.end()) na.push_back(ib);
-        auto& nb = isl.neighbours[ib];
-        if (std::find(nb.begin(), nb.end(), ia) == nb.end()) nb.push_back(ia);
-        int ends[2] = {e.a, e.b};
-        for (int re = 0; re < 2; ++re)
-            for (int ce = 0; ce < 2; ++ce) isl.block_terms[ends[re] * m + ends[ce]].push_back({jn, re, ce});
+        int ia = isl.pos[e.a], ib = isl.pos[e.b];
+        if (ia >= 0 && ib >= 0) {   // cache.rs:700-707: neighbours among the unknowns
+            auto& na = isl.neighbours[ia];
+            if (std::find(na.begin(), na.end(), ib) == na.end()) na.push_back(ib);
+            auto& nbr = isl.neighbours[ib];
+            if (std::find(nbr.begin(), nbr.end(), ia) == nbr.end()) nbr.push_back(ia);
+        }
+        int ends[2] = {ia, ib};
+        for (int re = 0; re < 2; ++re)        // cache.rs:708-720: no row or column for a kinematic end
+            for (int ce = 0; ce < 2; ++ce)
+                if (ends[re] >= 0 && ends[ce] >= 0) isl.block_terms[ends[re] * m + ends[ce]].push_back({jn, re, ce});
     }
 }
 
@@ -119,24 +134,25 @@ struct NewtonCtx {
 
     NewtonCtx(Island& i, const NewtonParams& p, double dt_, double warp_, const std::vector<std::array<double, 6>>& e)
         : isl(i), prm(p), dt(dt_), warp(warp_), ext(e) {
-        int m = (int)isl.bodies.size(), nj = (int)isl.joints.size();
+        int nb = (int)isl.bodies.size(), nj = (int)isl.joints.size();
         lanes.resize(LANES);
-        for (auto& l : lanes) l.resize(m, nj);
+        for (auto& l : lanes) l.resize(nb, isl.m, nj);
         jac.resize(nj);
-        dv.assign(m, {});
-        snap_mom.assign(m, {});
+        dv.assign(isl.m, {});
+        snap_mom.assign(nb, {});
     }
     int m() const { return isl.m; }
+    int nb() const { return (int)isl.bodies.size(); }
 
     // PRE + connection kernels + GATHER for one lane (accelerator/mod.rs:649-670)
     void dispatch(Lane& L, bool jacobian) {
-        int m = this->m();
-        for (int i = 0; i < m; ++i) {
+        const int nb = this->nb();
+        for (int i = 0; i < nb; ++i) {
             L.midpoint[i] = fn_pre(isl.bodies[i].pose, L.vmid[i], half);
             L.solve_vel[i] = L.vmid[i];
         }
         eval_joints(L, jacobian);
-        for (int i = 0; i < m; ++i) L.total[i] = ext[i];
+        for (int i = 0; i < nb; ++i) L.total[i] = ext[i];
         for (int jn = 0; jn < (int)isl.joints.size(); ++jn) {
             const Joint& e = isl.joints[jn];
             for (int s = 0; s < 6; ++s) L.total[e.a][s] = L.total[e.a][s] + L.conn[jn][0][s];
@@ -162,11 +178,12 @@ struct NewtonCtx {
     }
     void body_post(Lane& L) {
         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
-        for (int i = 0; i < m(); ++i) {
+        for (int p = 0; p < m(); ++p) {   // the unknowns only (body_post.rs:105-118)
+            const int i = isl.order[p];
             PostOut o = fn_body_post(L.midpoint[i], L.solve_vel[i], isl.bodies[i].inertia, snap_mom[i].data(),
                                      L.total[i].data(), half, floor2);
-            L.mass[i] = o.block;
-            for (int s = 0; s < 6; ++s) { L.rhs[i][s] = o.rhs[s]; L.scale[i][s] = o.scale[s]; }
+            L.mass[p] = o.block;
+            for (int s = 0; s < 6; ++s) { L.rhs[p][s] = o.rhs[s]; L.scale[p][s] = o.scale[s]; }
         }
     }
     static double max2(double a, double b) { return a > b ? a : b; }  // newton.rs:123-126
@@ -317,9 +334,10 @@ struct NewtonCtx {
         int m = this->m();
         const double huge = 1000000000.0;
         Lane& L0 = lanes[0];
-        for (int i = 0; i < m; ++i) L0.vmid[i] = world_velocity(isl.bodies[i]);
+        // every body's world velocity, a kinematic one's prescribed (newton.rs:357-376)
+        for (int i = 0; i < nb(); ++i) L0.vmid[i] = world_velocity(isl.bodies[i]);
         for (int l = 1; l < LANES; ++l)
-            for (int i = 0; i < m; ++i) lanes[l].vmid[i] = L0.vmid[i];
+            for (int i = 0; i < nb(); ++i) lanes[l].vmid[i] = L0.vmid[i];
         best = huge;
         std::vector<M16> best_vals = L0.vmid;
         converged = false;
@@ -356,8 +374,9 @@ struct NewtonCtx {
             std::vector<M16> base = L0.vmid;
             double alpha = 1.0;
             for (int l = 0; l < LANES; ++l) {
-                for (int i = 0; i < m; ++i) {
-                    M16 d6 = screw_from6(dv[i].data());
+                for (int p = 0; p < m; ++p) {
+                    const int i = isl.order[p];
+                    M16 d6 = screw_from6(dv[p].data());
                     lanes[l].vmid[i] = base[i] + d6.scale(alpha);
                 }
                 alpha = alpha / 2.0;
@@ -382,8 +401,9 @@ struct NewtonCtx {
                 has_carried = true;
                 carried = tnorm;
             } else {
-                for (int i = 0; i < m; ++i) {
-                    M16 d6 = screw_from6(dv[i].data());
+                for (int p = 0; p < m; ++p) {
+                    const int i = isl.order[p];
+                    M16 d6 = screw_from6(dv[p].data());
                     L0.vmid[i] = base[i] + d6.scale(0.0);
                 }
                 isl.stats.line_search_rejects++;
@@ -416,7 +436,8 @@ struct NewtonCtx {
     }
     void commit_span(double span, double half_, bool converged) {  // newton.rs:708-753
         const Lane& L0 = lanes[0];
-        for (int i = 0; i < m(); ++i) {
+        for (int p = 0; p < m(); ++p) {   // the unknowns only (newton.rs:704-753)
+            const int i = isl.order[p];
             Body& b = isl.bodies[i];
             const M16& vs = L0.vmid[i];
             M16 m_start = b.pose;
@@ -441,7 +462,7 @@ struct NewtonCtx {
             for (int k = 0; k < depth; ++k) span = span / 2.0;
             half = span / 2.0;
             isl.stats.spans++;
-            for (int i = 0; i < m(); ++i) screw_to6(world_momentum(isl.bodies[i]), snap_mom[i].data());
+            for (int i = 0; i < nb(); ++i) screw_to6(world_momentum(isl.bodies[i]), snap_mom[i].data());
             bool converged, assembled;
             double best;
             newton_solve(converged, best, assembled);

@@ -45,13 +45,19 @@ inline V3<double> solve3(const double a[3][3], const V3<double>& b) {  // inerti
 }
 
 // P = I·V in the body frame (inertia.rs:292-311, Full variant)
+// A kinematic body (Inertia::Kinematic, inertia.rs:103-151) is one of mass 0 here: its
+// momentum slot holds its velocity, apply and apply_inverse are the identity on the
+// components (inertia.rs:135, 321), and the integrators leave it alone.
+inline bool is_kinematic(const Inertia& in) { return in.mass == 0.0; }
 inline M16 inertia_apply(const Inertia& in, const M16& twist) {
     V3<double> v = screw_linear(twist), w = screw_angular(twist);
+    if (is_kinematic(in)) return screw_new(v, w);
     return screw_new(v.scale(in.mass), matvec3(in.t, w));
 }
 // V = I⁻¹·P (inertia.rs:313-333, Full variant)
 inline M16 inertia_apply_inverse(const Inertia& in, const M16& mom) {
     V3<double> f = screw_linear(mom), t = screw_angular(mom);
+    if (is_kinematic(in)) return screw_new(f, t);
     return screw_new(f.scale(1.0 / in.mass), solve3(in.t, t));
 }
 
@@ -350,6 +356,7 @@ inline void gravity_accumulate_for(int me, const std::vector<Charge>& front, dou
 // ───────────────────────── explicit integrators (crates/newton/src/integrator/explicit) ─────────────────────────
 // SymplecticEuler (symplectic_euler.rs:26-39); `wrench` in the body frame.
 inline void symplectic_euler_step(Body& b, const M16& wrench, double dt) {
+    if (is_kinematic(b.inertia)) return;   // symplectic_euler.rs:27-29
     M16 gyro = gyroscopic(b);
     b.momentum = b.momentum + (wrench + gyro).scale(dt);
     M16 v_new = body_velocity(b);
@@ -357,6 +364,7 @@ inline void symplectic_euler_step(Body& b, const M16& wrench, double dt) {
 }
 // ExplicitEuler (explicit_euler.rs:22-34)
 inline void explicit_euler_step(Body& b, const M16& wrench, double dt) {
+    if (is_kinematic(b.inertia)) return;   // explicit_euler.rs:22-24
     M16 v = body_velocity(b);
     M16 gyro = gyroscopic(b);
     b.pose = compose(twist_exp(v, dt), b.pose);
@@ -376,6 +384,7 @@ inline M16 rotation_part(const M16& m) {
 // LieEuler (lie_euler.rs:26-56): P += W·dt; δ = exp(dt·V(P)); P ← P transported by
 // rot(δ)⁻¹; M ← δ∘M — the momentum update by conjugation with the pose increment.
 inline void lie_euler_step(Body& b, const M16& wrench, double dt) {
+    if (is_kinematic(b.inertia)) return;   // lie_euler.rs:32-34
     b.momentum = b.momentum + wrench.scale(dt);
     M16 delta = twist_exp(body_velocity(b), dt);
     M16 drot = rotation_part(delta);
