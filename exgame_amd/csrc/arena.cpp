@@ -361,14 +361,16 @@ int exg_arena_admit(exg_arena* r, int n_bodies, const double* pose, const double
     if (int rc = exg::check_bodies("exg_arena_admit", n_bodies, pose, momentum, mass, inertia, size, n_islands, origin,
                                    n_joints, kind, params, G))
         return rc;
+    std::vector<int> md;
+    if (int rc = exg::dynamic_counts("exg_arena_admit", n_islands, island_start, mass, md)) return rc;
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
         const int j0 = jstart[i], ib0 = island_start[i];
         if (exg::island_class(island_start[i + 1] - ib0, jstart[i + 1] - j0) == exg::ISLAND_CLASS_LARGE)
             return afail(EXG_ERR_UNSUPPORTED, "exg_arena_admit: arenas take islands of up to 8 bodies and 16 joints "
                                               "(larger ones run in a world of their own)");
-        if (exg::island_topology_of(island_start[i + 1] - ib0, jstart[i + 1] - j0, kind + j0, a + j0, b + j0, ib0,
-                                    topo[i]) != 0)
+        if (exg::island_topology_of(island_start[i + 1] - ib0, md[i], jstart[i + 1] - j0, kind + j0, a + j0, b + j0,
+                                    ib0, topo[i]) != 0)
             return afail(EXG_ERR_UNSUPPORTED, "exg_arena_admit: an island exceeds the kernels' topology record");
     }
     long long nx = 0;

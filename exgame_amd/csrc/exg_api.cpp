@@ -86,12 +86,11 @@ int exg::check_bodies(const char* who, int n_bodies, const double* pose, const d
     };
     if (!std::isfinite(G)) return fail(EXG_ERR_INVALID, std::string(who) + ": G is not finite");
     for (long long i = 0; i < n_bodies; ++i) {
-        // Inertia::Kinematic (crates/newton/src/inertia.rs:12-45) has no mass: its body is
-        // a prescribed reference, which this library's solver does not model
-        if (mass[i] == 0.0)
-            return fail(EXG_ERR_UNSUPPORTED, std::string(who) + ": kinematic (zero-mass) body " + std::to_string(i) +
-                                                 " is not supported");
-        if (!(mass[i] > 0.0) || !std::isfinite(mass[i])) return bad("mass must be positive and finite", i);
+        // zero mass is Inertia::Kinematic (crates/newton/src/inertia.rs:103-151): a
+        // prescribed reference whose tensor is never read
+        const bool kinematic = mass[i] == 0.0;
+        if (!kinematic && (!(mass[i] > 0.0) || !std::isfinite(mass[i])))
+            return bad("mass must be positive (or zero: kinematic) and finite", i);
         const double* t = inertia + 9 * i;
         for (int k = 0; k < 9; ++k)
             if (!std::isfinite(t[k])) return bad("inertia tensor is not finite", i);
@@ -99,7 +98,7 @@ int exg::check_bodies(const char* who, int n_bodies, const double* pose, const d
         const double m1 = t[0], m2 = t[0] * t[4] - t[1] * t[3];
         const double m3 = t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6]) +
                           t[2] * (t[3] * t[7] - t[4] * t[6]);
-        if (!(m1 > 0.0 && m2 > 0.0 && m3 > 0.0)) return bad("inertia tensor is not positive definite", i);
+        if (!kinematic && !(m1 > 0.0 && m2 > 0.0 && m3 > 0.0)) return bad("inertia tensor is not positive definite", i);
         const double* p = pose + 8 * i;
         double r2 = 0.0;
         for (int k = 0; k < 8; ++k)
@@ -120,6 +119,22 @@ int exg::check_bodies(const char* who, int n_bodies, const double* pose, const d
     return EXG_OK;
 }
 
+int exg::dynamic_counts(const char* who, int n_islands, const int* island_start, const double* mass,
+                        std::vector<int>& md) {
+    md.assign(n_islands, 0);
+    for (int i = 0; i < n_islands; ++i) {
+        int d = 0;
+        for (int k = island_start[i]; k < island_start[i + 1]; ++k) {
+            if (mass[k] == 0.0) continue;
+            if (k - island_start[i] != d)
+                return fail(EXG_ERR_INVALID, std::string(who) + ": island " + std::to_string(i) + " has dynamic body " +
+                                                 std::to_string(k) + " after a kinematic one (kinematic bodies go last)");
+            ++d;
+        }
+        md[i] = d;
+    }
+    return EXG_OK;
+}
 
 struct exg_world {
     int B = 0, I = 0, J = 0;
@@ -222,11 +237,16 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     if (int rc = exg::check_bodies("exg_world_create", n_bodies, pose, momentum, mass, inertia, size, n_islands, origin,
                                    n_joints, kind, params, G))
         return rc;
+    std::vector<int> md;
+    if (int rc = exg::dynamic_counts("exg_world_create", n_islands, island_start, mass, md)) return rc;
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
-        const int j0 = jstart[i], b0 = island_start[i];
-        if (exg::island_topology_of(island_start[i + 1] - b0, jstart[i + 1] - j0, kind + j0, a + j0, b + j0, b0,
-                                    topo[i]) != 0)
+        const int j0 = jstart[i], b0 = island_start[i], mi = island_start[i + 1] - b0;
+        if (exg::island_class(mi, jstart[i + 1] - j0) == exg::ISLAND_CLASS_LARGE && md[i] != mi)
+            return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: island " + std::to_string(i) +
+                                                 ": kinematic bodies are taken in islands of up to 8 bodies and 16 "
+                                                 "joints");
+        if (exg::island_topology_of(mi, md[i], jstart[i + 1] - j0, kind + j0, a + j0, b + j0, b0, topo[i]) != 0)
             return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: an island exceeds the kernels' topology record");
     }
 

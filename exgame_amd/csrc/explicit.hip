@@ -92,6 +92,9 @@ __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs 
         }
     }
     for (int i = 0; i < m; ++i) {
+        // a kinematic body is a prescribed reference: the integrators leave it
+        // (symplectic_euler.rs:27-29, explicit_euler.rs:22-24, lie_euler.rs:32-34)
+        if (a.mass[b0 + i] == 0.0) continue;
         double inv[8], wbody[6];
         motor_rev(pose[i], inv);
         transport_co(total[i], inv, wbody);

@@ -16,13 +16,21 @@ int set_error(int code, const char* msg);
 int check_world(const char* who, int n_bodies, int n_islands, const int* island_start, int n_joints,
                 const int* kind, const int* a, const int* b, std::vector<int>& jstart, int& max_m, int& max_j);
 
-// Validates the state of a world's bodies, islands and joints: positive finite masses
-// (zero mass, a kinematic body, gives EXG_ERR_UNSUPPORTED), positive definite finite
-// inertia tensors, finite poses with a rotor part, finite momenta, sizes, origins,
-// joint parameters and G. Returns EXG_OK or the error (message recorded).
+// Validates the state of a world's bodies, islands and joints: finite masses, positive
+// or zero (zero: a kinematic body, Inertia::Kinematic, whose momentum slot holds its
+// body-frame velocity and whose inertia tensor is not read), positive definite finite
+// inertia tensors of the others, finite poses with a rotor part, finite momenta, sizes,
+// origins, joint parameters and G. Returns EXG_OK or the error (message recorded).
 int check_bodies(const char* who, int n_bodies, const double* pose, const double* momentum, const double* mass,
                  const double* inertia, const double* size, int n_islands, const double* origin, int n_joints,
                  const int* kind, const double* params, double G);
+
+// Dynamic bodies of each island (md[i]): the kernels take an island's kinematic bodies
+// after its dynamic ones, so that the unknowns of its system are its first md bodies
+// (the reference keeps the kinematic ones out of the system through an order map,
+// crates/newton/src/integrator/implicit/newton.rs:806-830). An island with a dynamic
+// body after a kinematic one gives EXG_ERR_INVALID. Returns EXG_OK or the error.
+int dynamic_counts(const char* who, int n_islands, const int* island_start, const double* mass, std::vector<int>& md);
 
 // Tables of one island that depend only on its joint graph (neighbour lists, assembly
 // terms, incidence, joints by kind, A's non-zero slab masks and the MFMA tile lists
@@ -40,10 +48,12 @@ struct IslandTopo {
     uint8_t pad[13];
 };
 static_assert(sizeof(IslandTopo) == 320, "IslandTopo is a 320-byte record");
-// m bodies, J joints with island-local ends ja/jb and kinds; `waves` and `per_wave`
-// from island_class_shape. Returns 0, or -1 when the island exceeds the record.
-int build_island_topology(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
-                          IslandTopo& t);
+// m bodies of which the first md are dynamic, J joints with island-local ends ja/jb and
+// kinds; `waves` and `per_wave` from island_class_shape. The system's tables (neighbour
+// lists, assembly terms, slab masks, tile lists) cover the dynamic bodies only; the
+// incidence lists cover every body. Returns 0, or -1 when the island exceeds the record.
+int build_island_topology(int m, int md, int J, const int* ja, const int* jb, const int* kind, int waves,
+                          int per_wave, IslandTopo& t);
 
 // The same tables for mechanism-sized islands (more than 8 bodies or 16 joints, up
 // to LARGE_MAXM bodies and LARGE_MAXJ joints), one record per such island. Neighbour
@@ -129,9 +139,9 @@ enum {
 int island_class(int m, int joints);   // -1 when no variant holds the island
 // Waves per workgroup and MFMA tiles per wave of a class's kernel (for its tile lists).
 void island_class_shape(int cls, int& waves, int& per_wave);
-// build_island_topology for island i of a world: bodies [b0, b0 + m), joints with
-// global body indices a/b; routes by island_class.
-int island_topology_of(int m, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t);
+// build_island_topology for island i of a world: bodies [b0, b0 + m), the first md of
+// them dynamic, joints with global body indices a/b; routes by island_class.
+int island_topology_of(int m, int md, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t);
 // One kernel launch; with a start or stop event the dispatch itself records them
 // (hipExtLaunchKernelGGL), otherwise a plain launch (which is also what stream
 // capture records).

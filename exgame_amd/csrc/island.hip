@@ -107,6 +107,7 @@ struct IslandShared {
     int term_start[MAXM * MAXM + 1], terms[4 * JD];
     int inc_start[MAXM + 1], inc[2 * JD];
     int ibcast[8];                          // [0]: a non-zero Newton correction was written; [1]: seeded flag out
+    int nd;                                 // dynamic bodies (the unknowns), the island's first nd
 };
 
 // Block-wide sum of one value per thread: a butterfly within each 64-lane wave,
@@ -141,7 +142,10 @@ struct IslandSolver {
     static constexpr bool SPLIT = !FLAT && NTH / 64 >= 4 && 8 * MAXM == 64 && 3 * MAXJ <= 64;
     static constexpr int LSR = SPLIT ? 3 : LS;
     int m, J, isl, first_body;
-    int n, n16;               // system size 6m and its 16-padded size
+    // dynamic bodies, the island's first md (the rest are kinematic references), read
+    // from LDS at each use rather than held in a register across the solve
+    __device__ int md() const { return uni(s.nd); }
+    int n, n16;               // system size 6·md and its 16-padded size
     bool mfma;                // Newton–Schulz products on MFMA (else VALU in the oracle's summation order)
     double dt, warp, half, span;
     long long xoff;
@@ -275,7 +279,7 @@ struct IslandSolver {
         const int nl = l1 - l0, t = tid();
         const int ll = t / LSTR, i = (t % LSTR) >> 3, j = t & 7;
         double r0 = 0.0, r1 = 0.0;
-        if (ll < nl && i < m) {
+        if (ll < nl && i < m) {   // kinematic bodies too (their rows are never read)
             const int lane = l0 + ll;
             double tot[6];
             for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
@@ -312,7 +316,7 @@ struct IslandSolver {
                 // one candidate per wave: its bodies' ratios are read from fixed lanes
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q)
-                    if (q < m) {
+                    if (q < md()) {
                         const double a0 = __longlong_as_double(
                             (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
                                              (int)(__double_as_longlong(r0) >> 32), q * 8) << 32) |
@@ -326,7 +330,7 @@ struct IslandSolver {
                     }
             } else {
                 const int base = (t & 63) & ~(LSTR - 1);
-                for (int q = 0; q < m; ++q) {
+                for (int q = 0; q < md(); ++q) {
                     const double a0 = __shfl(r0, base + q * 8, 64), a1 = __shfl(r1, base + q * 8, 64);
                     worst = max2(worst, a0);
                     worst = max2(worst, a1);
@@ -358,21 +362,23 @@ struct IslandSolver {
                 compose(s.pose[i], e, mid);
                 if (r == 0) {
                     double* vm = VMID(w, i);
-                    double* md = MID(w, i);
+                    double* mw = MID(w, i);
 #pragma unroll
                     for (int k = 0; k < 6; ++k) vm[k] = v[k];
 #pragma unroll
-                    for (int k = 0; k < 8; ++k) md[k] = mid[k];
+                    for (int k = 0; k < 8; ++k) mw[k] = mid[k];
                 }
-                double brow[6], p[6];
-                body_post_row_momentum(mid, v, s.massv[i], s.tens[i], r < 6 ? r : 5, brow, p);
-                if (r == 0) {
-                    double* rh = RHS(w, i);
+                if (i < md()) {
+                    double brow[6], p[6];
+                    body_post_row_momentum(mid, v, s.massv[i], s.tens[i], r < 6 ? r : 5, brow, p);
+                    if (r == 0) {
+                        double* rh = RHS(w, i);
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) rh[k] = p[k];
+                        for (int k = 0; k < 6; ++k) rh[k] = p[k];
+                    }
+                    if (w == 0 && mass_of_candidate0 && r < 6)
+                        for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = brow[c];
                 }
-                if (w == 0 && mass_of_candidate0 && r < 6)
-                    for (int c = 0; c < 6; ++c) s.mass0[i][r * 6 + c] = brow[c];
             }
         } else if constexpr (MAXJ > 0) {
             const int q = t - 64 * nl;
@@ -403,7 +409,7 @@ struct IslandSolver {
         if (w >= nl) return;
         const int i = (t & 63) >> 3, r = t & 7;
         double r0 = 0.0, r1 = 0.0;
-        if (i < m) {
+        if (i < md()) {
             double tot[6];
             for (int k = 0; k < 6; ++k) tot[k] = s.ext[i][k];
             for (int q = s.inc_start[i]; q < s.inc_start[i + 1]; ++q) {
@@ -436,7 +442,7 @@ struct IslandSolver {
         double worst = 0.0;
 #pragma unroll
         for (int q = 0; q < MAXM; ++q)
-            if (q < m) {
+            if (q < md()) {
                 const double a0 = __longlong_as_double(
                     (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane(
                                      (int)(__double_as_longlong(r0) >> 32), q * 8) << 32) |
@@ -517,7 +523,7 @@ struct IslandSolver {
             if (e >= n * n) break;
             int row = e / n, col = e % n, i = row / 6, j = col / 6, c = col % 6;
             double acc = 0.0 * out[row * LD + col];
-            int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : m;
+            int k0 = sparse ? s.nbr_start[i] : 0, k1 = sparse ? s.nbr_start[i + 1] : md();
             for (int q = k0; q < k1; ++q) {
                 int k = sparse ? s.nbr[q] : q;
                 double diag = (two_minus && k == j) ? 1.0 : 0.0;
@@ -696,12 +702,12 @@ struct IslandSolver {
     }
     __device__ void seed_transpose() {  // newton.rs:209-244
         // per block row i: Σ|A[i, ·]| and per block column i: Σ|A[·, i]|
-        if (tid() < 2 * m) {
-            int i = tid() % m;
-            bool col = tid() >= m;
+        if (tid() < 2 * md()) {
+            int i = tid() % md();
+            bool col = tid() >= md();
             double sum = 0.0;
             const double* Am = A();
-            for (int j = 0; j < m; ++j)
+            for (int j = 0; j < md(); ++j)
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c)
                         sum += fabs(col ? Am[(6 * j + r) * LD + 6 * i + c] : Am[(6 * i + r) * LD + 6 * j + c]);
@@ -710,9 +716,9 @@ struct IslandSolver {
         __syncthreads();
         if (tid() == 0) {
             double one = 0.0, inf = 0.0;
-            for (int i = 0; i < m; ++i) {
+            for (int i = 0; i < md(); ++i) {
                 inf = max2(inf, s.seedsum[i]);
-                one = max2(one, s.seedsum[m + i]);
+                one = max2(one, s.seedsum[md() + i]);
             }
             s.bcast[0] = 1.0 / (one * inf);
         }
@@ -760,7 +766,7 @@ struct IslandSolver {
             const bool live = t < 4 * n;
             double sum = 0.0;
             if (live)
-                for (int j = part; j < m; j += 4) {
+                for (int j = part; j < md(); j += 4) {
                     const double* xb = Xm + row * LD + 6 * j;
                     double sb = 0.0;
                     for (int c = 0; c < 6; ++c) sb = sb + xb[c] * s.rhs0[j][c];
@@ -779,7 +785,7 @@ struct IslandSolver {
         for (int t = tid(); t < n; t += NTH) {
             int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
-            for (int j = 0; j < m; ++j) {
+            for (int j = 0; j < md(); ++j) {
                 const double* xb = Xm + t * LD + 6 * j;
                 double sum = 0.0;
                 for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.rhs0[j][c];
@@ -803,7 +809,16 @@ struct IslandSolver {
         for (int t = tid(); t < m; t += NTH)
             world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.vmid0[t]);
         __syncthreads();
-        for (int t = tid(); t < m * 6; t += NTH) s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+        // kinematic bodies: the line search's base holds their prescribed velocity and
+        // their dv is −0, the additive identity, so that every candidate base + α·dv is
+        // that velocity bit for bit (the matvec writes the unknowns' rows only)
+        for (int t = tid(); t < m * 6; t += NTH) {
+            s.best[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+            if (t >= md() * 6) {
+                s.base[t / 6][t % 6] = s.vmid0[t / 6][t % 6];
+                s.dv[t / 6][t % 6] = -0.0;
+            }
+        }
         __syncthreads();
         best = huge;
         converged = false;
@@ -891,7 +906,7 @@ struct IslandSolver {
                         for (int t = tid(); t < m * 8; t += NTH) s.mid0[t / 8][t % 8] = MID(as, t / 8)[t % 8];
                     }
                     // the accepted lane's mass block, recomputed bit-identically from its inputs
-                    for (int t = tid(); t < m; t += NTH) {
+                    for (int t = tid(); t < md(); t += NTH) {
                         double block[6][6], rr[6], sc[2];
                         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
                         body_post(MID(as, t), VMID(as, t), s.massv[t], s.tens[t], s.snap[t], TOTAL(as, t), half, floor2,
@@ -904,7 +919,7 @@ struct IslandSolver {
                 has_carried = true;
                 carried = tnorm;
             } else {
-                for (int t = tid(); t < m * 6; t += NTH) s.vmid0[t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
+                for (int t = tid(); t < md() * 6; t += NTH) s.vmid0[t / 6][t % 6] = s.base[t / 6][t % 6] + s.dv[t / 6][t % 6] * 0.0;
                 __syncthreads();
                 st_reject++;
                 break;
@@ -936,7 +951,7 @@ struct IslandSolver {
         double mn = 0.0;
         {
             double part = 0.0;
-            for (int e = tid(); e < m * 36; e += NTH) {
+            for (int e = tid(); e < md() * 36; e += NTH) {
                 double v = s.mass0[e / 36][e % 36];
                 part = part + v * v;
             }
@@ -1001,7 +1016,7 @@ struct IslandSolver {
             }
             if (converged || finite) {
                 if (!converged) st_uncommit++;
-                for (int t = tid(); t < m; t += NTH)
+                for (int t = tid(); t < md(); t += NTH)   // the unknowns only (newton.rs:704-753)
                     commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.vmid0[t], s.snap[t], span, half,
                                 converged);
                 __syncthreads();
@@ -1051,7 +1066,11 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     }
     const long long xoff = a.xacc_off[isl];
     constexpr int LD = IslandShared<MAXM, MAXJ, NTH, JF>::LDM;
-    const int n = 6 * m;
+    // the unknowns: the dynamic bodies, which come first (the host checks the order)
+    int md = 0;
+    for (int i = 0; i < m; ++i) md += a.mass[b0 + i] != 0.0 ? 1 : 0;
+    md = uni(md);
+    const int n = 6 * md;
     for (int e = threadIdx.x; e < IslandShared<MAXM, MAXJ, NTH, JF>::NB; e += NTH) {
         s.A[e] = 0.0;
         s.X[e] = 0.0;
@@ -1078,6 +1097,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         if (t < 2 * SH::JD) s.inc[t] = tp.inc[t];
         if (t < SH::JD) s.jorder[t] = tp.jorder[t];
         if (t < SH::MAXT) s.kmask[t] = tp.kmask[t];
+        if (t == 0) s.nd = md;
         if (t < 2 * SH::WAVES * SH::PER_WAVE) {
             const int k = t / (SH::WAVES * SH::PER_WAVE), w = (t / SH::PER_WAVE) % SH::WAVES, u = t % SH::PER_WAVE;
             const int e = tp.tlist[k][w][u];

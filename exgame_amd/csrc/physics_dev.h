@@ -139,11 +139,16 @@ __host__ __device__ inline void inertia_apply_inverse(const S& mass, const S a[9
     v[5] = (c02 * b[0] + c12 * b[1] + c22 * b[2]) / det;
 }
 
-// World twist Ad_M(I⁻¹P) and world momentum Ad*_M P (body.rs:145-197).
+// World twist Ad_M(I⁻¹P) and world momentum Ad*_M P (body.rs:145-197). A kinematic body
+// (mass 0) has the identity for I⁻¹: its momentum slot holds its body-frame velocity
+// (crates/newton/src/inertia.rs:103-151).
 template <class S>
 __host__ __device__ inline void world_velocity(const S pose[8], const S& mass, const S t9[9], const S mom[6], S out[6]) {
     S vb[6];
-    inertia_apply_inverse(mass, t9, mom, vb);
+    if (mass == cst<S>(0.0))
+        for (int k = 0; k < 6; ++k) vb[k] = mom[k];
+    else
+        inertia_apply_inverse(mass, t9, mom, vb);
     transport_contra(vb, pose, out);
 }
 template <class S>

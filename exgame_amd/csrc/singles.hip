@@ -590,6 +590,17 @@ __global__ __launch_bounds__(SB) void singles_step_kernel(IslandArgs a) {
 #pragma unroll
     for (int q = 0; q < 9; ++q) sg.tens[q] = a.tens[(size_t)b * 9 + q];
     sg.mass = a.mass[b];
+    if (sg.mass == 0.0) {
+        // a kinematic body alone: no unknowns, so the step is one span whose first
+        // Newton iteration finds the empty residual converged, and nothing is committed
+        // (newton.rs:806-830, 704-753); its pose and velocity stay as they are
+        StepCounts n;
+        n.spans = 1;
+        n.newton = 1;
+        n.ns = n.reseed = n.reject = n.uncommit = n.nonfinite = 0;
+        finish_island(a, isl, b, sg.pose, sg.mom, sg.mass, a.seeded[isl] != 0, n, true);
+        return;
+    }
     sg.an = 0.0;
     sg.load_x(a.xacc_off[isl]);
     sg.dt = a.dt;

@@ -20,21 +20,23 @@
 
 namespace exg {
 
-int build_island_topology(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
-                          IslandTopo& t) {
+int build_island_topology(int m, int md, int J, const int* ja, const int* jb, const int* kind, int waves,
+                          int per_wave, IslandTopo& t) {
     std::memset(&t, 0, sizeof t);
-    if (m < 1 || m > TOPO_MAXM || J < 0 || J > TOPO_MAXJ || waves < 1 || waves > TOPO_WAVES || per_wave < 1 ||
-        per_wave > TOPO_PER_WAVE)
+    if (m < 1 || m > TOPO_MAXM || md < 0 || md > m || J < 0 || J > TOPO_MAXJ || waves < 1 || waves > TOPO_WAVES ||
+        per_wave < 1 || per_wave > TOPO_PER_WAVE)
         return -1;
     for (int j = 0; j < J; ++j)
         if (ja[j] < 0 || ja[j] >= m || jb[j] < 0 || jb[j] >= m || kind[j] < 0 || kind[j] > 3) return -1;
-    // neighbour lists
+    // neighbour lists: the blocks of A, between dynamic bodies only (a kinematic body
+    // has no row or column; its rows are empty lists)
     int lists[TOPO_MAXM][TOPO_MAXM], cnt[TOPO_MAXM];
     for (int i = 0; i < m; ++i) {
         lists[i][0] = i;
-        cnt[i] = 1;
+        cnt[i] = i < md ? 1 : 0;
     }
     auto add = [&](int i, int k) {
+        if (i >= md || k >= md) return;
         for (int q = 0; q < cnt[i]; ++q)
             if (lists[i][q] == k) return;
         lists[i][cnt[i]++] = k;
@@ -58,7 +60,8 @@ int build_island_topology(int m, int J, const int* ja, const int* jb, const int*
             const int ends[2] = {ja[j], jb[j]};
             for (int re = 0; re < 2; ++re)
                 for (int ce = 0; ce < 2; ++ce)
-                    if (ends[re] == bi && ends[ce] == bj) t.terms[o++] = (uint8_t)((j << 2) | (re << 1) | ce);
+                    if (bi < md && bj < md && ends[re] == bi && ends[ce] == bj)
+                        t.terms[o++] = (uint8_t)((j << 2) | (re << 1) | ce);
         }
     }
     t.term_start[m * m] = (uint8_t)o;
@@ -87,7 +90,7 @@ int build_island_topology(int m, int J, const int* ja, const int* jb, const int*
     // tile lists: round robin for the dense product; for the sparse one, largest
     // count of non-zero slabs first, each to the least-loaded wave (fewest tiles on
     // a tie, then the lowest wave); round robin again if a wave would overflow
-    const int T = (6 * m + 15) / 16, NTT = T * T;
+    const int T = (6 * md + 15) / 16, NTT = T * T;
     int rr[TOPO_WAVES][TOPO_PER_WAVE], rrn[TOPO_WAVES] = {};
     for (int tt = 0; tt < NTT; ++tt) {
         const int w = tt % waves;
@@ -198,7 +201,7 @@ long long large_workspace_doubles(int m, int J) {
     return 4 * n16 * n16 + 288LL * J;
 }
 
-int island_topology_of(int m, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t) {
+int island_topology_of(int m, int md, int J, const int* kind, const int* a, const int* b, int b0, IslandTopo& t) {
     const int cls = island_class(m, J);
     if (cls < 0) return -1;
     if (cls == ISLAND_CLASS_M1 || cls == ISLAND_CLASS_LARGE) {   // no fixed record (large: LargeTopo)
@@ -213,7 +216,7 @@ int island_topology_of(int m, int J, const int* kind, const int* a, const int* b
         la[j] = a[j] - b0;
         lb[j] = b[j] - b0;
     }
-    return build_island_topology(m, J, la, lb, kind, waves, per_wave, t);
+    return build_island_topology(m, md, J, la, lb, kind, waves, per_wave, t);
 }
 
 }  // namespace exg

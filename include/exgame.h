@@ -91,10 +91,15 @@ typedef struct exg_world exg_world;
  *               grouped island by island in declaration order), params[J*10]
  *   G: gravitational constant (gravity off when gravity_on == 0)
  * Returns EXG_ERR_UNSUPPORTED for an island beyond this build's on-chip size limits. */
-/* The state is validated: EXG_ERR_INVALID for a non-positive or non-finite mass, an
- * inertia tensor that is not finite and positive definite, a non-finite pose,
- * momentum, size, origin, joint parameter or G; EXG_ERR_UNSUPPORTED for a zero-mass
- * (kinematic, crates/newton/src/inertia.rs:12-45) body. */
+/* The state is validated: EXG_ERR_INVALID for a negative or non-finite mass, an
+ * inertia tensor of a body with mass that is not finite and positive definite, a
+ * non-finite pose, momentum, size, origin, joint parameter or G.
+ * Mass 0 is a kinematic body (Inertia::Kinematic, crates/newton/src/inertia.rs:103-151):
+ * its momentum slot holds its body-frame velocity, it is never moved, and its joints
+ * act on their other ends with that velocity (newton.rs:806-830). Its tensor is not
+ * read (finite values required). An island's kinematic bodies come after its dynamic
+ * ones (EXG_ERR_INVALID otherwise); islands of the mechanism-sized class (more than 8
+ * bodies or 16 joints) with a kinematic body give EXG_ERR_UNSUPPORTED. */
 int exg_world_create(int n_bodies, const double* pose, const double* momentum, const double* mass,
                      const double* inertia, const double* size, int n_islands, const int* island_start,
                      const double* origin, int n_joints, const int* kind, const int* a, const int* b,

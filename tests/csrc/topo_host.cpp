@@ -9,12 +9,17 @@ int island_class(int, int) { return -1; }
 void island_class_shape(int, int& waves, int& per_wave) { waves = per_wave = 1; }
 }  // namespace exg
 
-extern "C" int tp_build(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
-                        unsigned char* out) {
+extern "C" int tp_build_md(int m, int md, int J, const int* ja, const int* jb, const int* kind, int waves,
+                           int per_wave, unsigned char* out) {
     exg::IslandTopo t;
-    const int rc = exg::build_island_topology(m, J, ja, jb, kind, waves, per_wave, t);
+    const int rc = exg::build_island_topology(m, md, J, ja, jb, kind, waves, per_wave, t);
     std::memcpy(out, &t, sizeof t);
     return rc;
+}
+// every body dynamic
+extern "C" int tp_build(int m, int J, const int* ja, const int* jb, const int* kind, int waves, int per_wave,
+                        unsigned char* out) {
+    return tp_build_md(m, m, J, ja, jb, kind, waves, per_wave, out);
 }
 extern "C" int tp_size() { return (int)sizeof(exg::IslandTopo); }
 

@@ -33,13 +33,13 @@ def tp():
     return L
 
 
-def build(L, m, ja, jb, kind, waves, per_wave):
+def build(L, m, ja, jb, kind, waves, per_wave, md=None):
     J = len(ja)
     buf = np.zeros(320, dtype=np.uint8)
     arr = [np.ascontiguousarray(x, dtype=np.int32) for x in (ja, jb, kind)]
     ip = C.POINTER(C.c_int)
-    rc = L.tp_build(m, J, *[x.ctypes.data_as(ip) for x in arr], waves, per_wave,
-                    buf.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    rc = L.tp_build_md(m, m if md is None else md, J, *[x.ctypes.data_as(ip) for x in arr], waves, per_wave,
+                       buf.ctypes.data_as(C.POINTER(C.c_ubyte)))
     out, off = {}, 0
     for name, dt, cnt in FIELDS:
         size = np.dtype(dt).itemsize * cnt
@@ -273,3 +273,44 @@ def test_large_topology_rejects_oversize(tp):
     assert build_large(tp, 10, [0] * 129, [1] * 129)[0] == -1
     assert build_large(tp, 10, [0], [10])[0] == -1   # end outside the island
     assert build_large(tp, 10, [3], [3])[0] == -1    # a joint from a body to itself
+
+
+@pytest.mark.parametrize("m,k", [(2, 1), (5, 1), (8, 1), (8, 2), (4, 4)])
+def test_kinematic_bodies_have_no_rows_or_columns(tp, m, k):
+    # the last k bodies are kinematic (references, not unknowns: newton.rs:806-830):
+    # the system's tables are those of the dynamic bodies alone with the joints among
+    # them; the incidence still lists every joint at both ends
+    md = m - k
+    rng = np.random.default_rng(m * 10 + k)
+    ja = list(range(md - 1)) if md > 1 else []
+    jb = list(range(1, md)) if md > 1 else []
+    kind = [int(x) for x in rng.integers(0, 4, len(ja))]
+    dyn = (list(ja), list(jb), list(kind))
+    for q in range(k):   # joints to the kinematic bodies last
+        ja.append(md + q); jb.append(int(rng.integers(0, md)) if md else (md + q + 1) % m); kind.append(q % 4)
+    waves, per_wave = shape_of(m, len(ja))
+    rc, t = build(tp, m, ja, jb, kind, waves, per_wave, md=md)
+    assert rc == 0
+    if md > 0:
+        rc0, t0 = build(tp, md, *dyn, waves, per_wave)
+        assert rc0 == 0
+        for f in ("kmask", "tcount", "tlist"):
+            np.testing.assert_array_equal(t[f], t0[f], err_msg=f)
+        np.testing.assert_array_equal(t["nbr_start"][:md + 1], t0["nbr_start"][:md + 1])
+        np.testing.assert_array_equal(t["nbr"][:t0["nbr_start"][md]], t0["nbr"][:t0["nbr_start"][md]])
+    else:
+        assert not t["kmask"].any() and not t["tcount"].any()
+    # kinematic rows are empty lists; no term names a kinematic end
+    assert (t["nbr_start"][md:m + 1] == t["nbr_start"][md]).all()
+    for b_ in range(m * m):
+        bi, bj = divmod(b_, m)
+        n_terms = t["term_start"][b_ + 1] - t["term_start"][b_]
+        if bi >= md or bj >= md:
+            assert n_terms == 0
+    for code in t["terms"][:t["term_start"][m * m]]:
+        j, re, ce = code >> 2, (code >> 1) & 1, code & 1
+        ends = (ja[j], jb[j])
+        assert ends[re] < md and ends[ce] < md
+    # incidence: every (joint, end) once, kinematic ends included
+    inc = sorted(int(c) for c in t["inc"][:t["inc_start"][m]])
+    assert inc == sorted([j << 1 for j in range(len(ja))] + [(j << 1) | 1 for j in range(len(ja))])

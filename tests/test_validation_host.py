@@ -20,7 +20,7 @@ def bad(scene, **fields):
 @pytest.mark.parametrize("field,idx,val,code", [
     ("mass", 3, -1.0, 6),          # negative mass: invalid
     ("mass", 2, np.nan, 6),
-    ("mass", 1, 0.0, 7),           # zero mass = a kinematic body: unsupported
+    ("mass", 1, 0.0, 6),           # a kinematic body (mass 0) before dynamic ones in its island
     ("inertia", 9 * 2 + 4, -0.5, 6),   # not positive definite
     ("inertia", 0, np.inf, 6),
     ("pose", 8 * 1 + 5, np.nan, 6),
@@ -65,3 +65,27 @@ def test_create_refuses_islands_beyond_the_large_record():
     with pytest.raises(_lib.ExgError) as e:
         DeviceWorld(s)
     assert e.value.code == 7 and "128" in str(e.value)
+
+
+def test_kinematic_bodies_last_in_their_islands_pass_validation():
+    # anchored chains: each island's anchor (mass 0, zero tensor) is its last body; the
+    # host checks pass and creation goes on to the device (absent here: a backend error)
+    s = scenes.anchored_chains(n_chains=3, m=8, seed=1)
+    assert (s.mass == 0.0).sum() == 3
+    with pytest.raises(_lib.ExgError) as e:
+        DeviceWorld(s)
+    assert e.value.code == 5, str(e.value)
+
+
+def test_kinematic_bodies_first_are_refused():
+    s = scenes.anchored_chains(n_chains=2, m=4, seed=1, kinematic_first=True)
+    with pytest.raises(_lib.ExgError) as e:
+        DeviceWorld(s)
+    assert e.value.code == 6 and "kinematic bodies go last" in str(e.value)
+
+
+def test_kinematic_body_in_a_mechanism_sized_island_is_unsupported():
+    s = scenes.anchored_chains(n_chains=1, m=12, seed=1)
+    with pytest.raises(_lib.ExgError) as e:
+        DeviceWorld(s)
+    assert e.value.code == 7 and "kinematic" in str(e.value)
