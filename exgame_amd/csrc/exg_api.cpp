@@ -242,10 +242,6 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
     std::vector<exg::IslandTopo> topo(n_islands);
     for (int i = 0; i < n_islands; ++i) {
         const int j0 = jstart[i], b0 = island_start[i], mi = island_start[i + 1] - b0;
-        if (exg::island_class(mi, jstart[i + 1] - j0) == exg::ISLAND_CLASS_LARGE && md[i] != mi)
-            return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: island " + std::to_string(i) +
-                                                 ": kinematic bodies are taken in islands of up to 8 bodies and 16 "
-                                                 "joints");
         if (exg::island_topology_of(mi, md[i], jstart[i + 1] - j0, kind + j0, a + j0, b + j0, b0, topo[i]) != 0)
             return fail(EXG_ERR_UNSUPPORTED, "exg_world_create: an island exceeds the kernels' topology record");
     }
@@ -359,7 +355,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
                 la[q] = a[jstart[i] + q] - island_start[i];
                 lb[q] = b[jstart[i] + q] - island_start[i];
             }
-            if (exg::build_large_topology(mm, jj, la.data(), lb.data(), lt[i]) != 0) {
+            if (exg::build_large_topology(mm, md[i], jj, la.data(), lb.data(), lt[i]) != 0) {
                 fail(EXG_ERR_UNSUPPORTED, "exg_world_create: island " + std::to_string(i) + " exceeds the large-island record");
                 return bail(EXG_ERR_UNSUPPORTED);
             }

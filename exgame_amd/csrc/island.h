@@ -68,8 +68,10 @@ struct LargeTopo {
     uint16_t inc_start[LARGE_MAXM + 1], inc[2 * LARGE_MAXJ];       // joint << 1 | end
     uint32_t kmask[LARGE_MAXT][3];                        // per 16-row tile: its non-zero 4-column slabs
 };
-// m bodies, J joints with island-local ends; returns 0, or -1 beyond the record.
-int build_large_topology(int m, int J, const int* ja, const int* jb, LargeTopo& t);
+// m bodies of which the first md are dynamic (A's blocks, and so the neighbour lists,
+// assembly terms and slab masks, cover those; the incidence covers every body), J
+// joints with island-local ends; returns 0, or -1 beyond the record.
+int build_large_topology(int m, int md, int J, const int* ja, const int* jb, LargeTopo& t);
 // Doubles of device workspace one large island of m bodies and J joints uses: the
 // system matrices A, X, X', T (n16 × n16 each, n16 = 6m rounded up to 16) and the
 // joint Jacobians (J × 288).

@@ -91,6 +91,7 @@ struct LargeSolver {
     LargeShared& s;
     const IslandArgs& a;
     int m, J, n, n16, LD, nnz;
+    int md;   // dynamic bodies (the unknowns), the island's first md; the rest are kinematic references
     double dt, warp, half, span;
     long long xoff;
     double *A, *X, *Xn, *T, *jac;   // workspace
@@ -164,7 +165,7 @@ struct LargeSolver {
         __syncthreads();
         if (t == 0) {
             double worst = 0.0;
-            for (int q = 0; q < m; ++q) {
+            for (int q = 0; q < md; ++q) {
                 worst = max2(worst, s.ratio[q][0]);
                 worst = max2(worst, s.ratio[q][1]);
             }
@@ -270,11 +271,11 @@ struct LargeSolver {
         return before;
     }
     __device__ void seed_transpose() {  // newton.rs:209-244
-        if (tid() < 2 * m) {
-            const int i = tid() % m;
-            const bool col = tid() >= m;
+        if (tid() < 2 * md) {
+            const int i = tid() % md;
+            const bool col = tid() >= md;
             double sum = 0.0;
-            for (int j = 0; j < m; ++j)
+            for (int j = 0; j < md; ++j)
                 for (int r = 0; r < 6; ++r)
                     for (int c = 0; c < 6; ++c)
                         sum += fabs(col ? A[(size_t)(6 * j + r) * LD + 6 * i + c] : A[(size_t)(6 * i + r) * LD + 6 * j + c]);
@@ -283,9 +284,9 @@ struct LargeSolver {
         __syncthreads();
         if (tid() == 0) {
             double one = 0.0, inf = 0.0;
-            for (int i = 0; i < m; ++i) {
+            for (int i = 0; i < md; ++i) {
                 inf = max2(inf, s.seedsum[i]);
-                one = max2(one, s.seedsum[m + i]);
+                one = max2(one, s.seedsum[md + i]);
             }
             s.bcast[0] = 1.0 / (one * inf);
         }
@@ -324,7 +325,7 @@ struct LargeSolver {
         if (t < n) {
             const int i = t / 6, r = t % 6;
             double acc = 0.0 * s.dv[i][r];
-            for (int j = 0; j < m; ++j) {
+            for (int j = 0; j < md; ++j) {
                 const double* xb = X + (size_t)t * LD + 6 * j;
                 double sum = 0.0;
                 for (int c = 0; c < 6; ++c) sum = sum + xb[c] * s.sl[0].rhs[j][c];
@@ -343,7 +344,16 @@ struct LargeSolver {
         for (int t = tid(); t < m; t += NT)
             world_velocity(s.pose[t], s.massv[t], s.tens[t], s.mom[t], s.sl[0].vmid[t]);
         __syncthreads();
-        for (int t = tid(); t < m * 6; t += NT) s.best[t / 6][t % 6] = s.sl[0].vmid[t / 6][t % 6];
+        // kinematic bodies: the line search's base holds their prescribed velocity and
+        // their dv is −0, the additive identity, so that every candidate base + α·dv is
+        // that velocity bit for bit (the matvec writes the unknowns' rows only)
+        for (int t = tid(); t < m * 6; t += NT) {
+            s.best[t / 6][t % 6] = s.sl[0].vmid[t / 6][t % 6];
+            if (t >= md * 6) {
+                s.base[t / 6][t % 6] = s.sl[0].vmid[t / 6][t % 6];
+                s.dv[t / 6][t % 6] = -0.0;
+            }
+        }
         __syncthreads();
         best = huge;
         converged = false;
@@ -420,7 +430,7 @@ struct LargeSolver {
                     }
                     for (int t = tid(); t < m * 8; t += NT) s.sl[0].mid[t / 8][t % 8] = s.sl[1].mid[t / 8][t % 8];
                     // the accepted lane's mass block, recomputed bit-identically from its inputs
-                    for (int t = tid(); t < m; t += NT) {
+                    for (int t = tid(); t < md; t += NT) {
                         double block[6][6], rr[6], sc[2];
                         const double floor2 = (1.0 / 1000000000.0) * (1.0 / 1000000000.0);
                         body_post(s.sl[1].mid[t], s.sl[1].vmid[t], s.massv[t], s.tens[t], s.snap[t], s.sl[1].total[t],
@@ -464,7 +474,7 @@ struct LargeSolver {
         if (!assembled) return false;
         const double an = reduce_sq(A, false);
         double part = 0.0;
-        for (int e = tid(); e < m * 36; e += NT) {
+        for (int e = tid(); e < md * 36; e += NT) {
             const double v = s.mass0[e / 36][e % 36];
             part = part + v * v;
         }
@@ -531,7 +541,7 @@ struct LargeSolver {
             }
             if (converged || finite) {
                 if (!converged) st_uncommit++;
-                for (int t = tid(); t < m; t += NT)
+                for (int t = tid(); t < md; t += NT)   // the unknowns only (newton.rs:704-753)
                     commit_body(s.pose[t], s.mom[t], s.massv[t], s.tens[t], s.sl[0].vmid[t], s.snap[t], span, half,
                                 converged);
                 __syncthreads();
@@ -591,7 +601,11 @@ __global__ __launch_bounds__(NT) void island_step_kernel_large(IslandArgs a) {
     LargeSolver sv{s, a};
     sv.m = m;
     sv.J = J;
-    sv.n = 6 * m;
+    // the unknowns: the dynamic bodies, which come first (the host checks the order)
+    int md = 0;
+    for (int i = 0; i < m; ++i) md += a.mass[b0 + i] != 0.0 ? 1 : 0;
+    sv.md = uni(md);
+    sv.n = 6 * sv.md;
     sv.n16 = ((sv.n + 15) / 16) * 16;
     sv.LD = sv.n16;
     sv.dt = a.dt;

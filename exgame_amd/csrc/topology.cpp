@@ -135,15 +135,17 @@ int build_island_topology(int m, int md, int J, const int* ja, const int* jb, co
     return 0;
 }
 
-int build_large_topology(int m, int J, const int* ja, const int* jb, LargeTopo& t) {
+int build_large_topology(int m, int md, int J, const int* ja, const int* jb, LargeTopo& t) {
     std::memset(&t, 0, sizeof t);
-    if (m < 1 || m > LARGE_MAXM || J < 0 || J > LARGE_MAXJ) return -1;
+    if (m < 1 || m > LARGE_MAXM || md < 0 || md > m || J < 0 || J > LARGE_MAXJ) return -1;
     for (int j = 0; j < J; ++j)
         if (ja[j] < 0 || ja[j] >= m || jb[j] < 0 || jb[j] >= m || ja[j] == jb[j]) return -1;
     // neighbour lists: body i first, then the other ends of its joints in joint order
+    // (between dynamic bodies only: a kinematic body has no row or column in A)
     std::vector<std::vector<int>> lists(m);
-    for (int i = 0; i < m; ++i) lists[i].push_back(i);
+    for (int i = 0; i < md; ++i) lists[i].push_back(i);
     auto add = [&](int i, int k) {
+        if (i >= md || k >= md) return;
         for (int x : lists[i])
             if (x == k) return;
         lists[i].push_back(k);

@@ -25,11 +25,15 @@ extern "C" int tp_size() { return (int)sizeof(exg::IslandTopo); }
 
 // the mechanism-sized record: tp_build_large fills one LargeTopo; tp_large_layout
 // writes the byte offsets of its eight fields and its size
-extern "C" int tp_build_large(int m, int J, const int* ja, const int* jb, unsigned char* out) {
+extern "C" int tp_build_large_md(int m, int md, int J, const int* ja, const int* jb, unsigned char* out) {
     exg::LargeTopo t;
-    const int rc = exg::build_large_topology(m, J, ja, jb, t);
+    const int rc = exg::build_large_topology(m, md, J, ja, jb, t);
     std::memcpy(out, &t, sizeof t);
     return rc;
+}
+// every body dynamic
+extern "C" int tp_build_large(int m, int J, const int* ja, const int* jb, unsigned char* out) {
+    return tp_build_large_md(m, m, J, ja, jb, out);
 }
 extern "C" void tp_large_layout(long long* out) {
     out[0] = offsetof(exg::LargeTopo, nbr_start);

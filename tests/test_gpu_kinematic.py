@@ -131,3 +131,28 @@ def test_mixed_precision_with_anchors_stays_within_the_gate():
     dev.step(20, 1e-2, 1.0)
     orc.step(20, 1e-2, 1.0)
     assert rel(dev.state()["position"], orc.state()["position"]) < 1e-9
+
+
+@pytest.mark.parametrize("m", [12, 20])
+def test_anchored_long_chains_on_the_mechanism_sized_kernel(m):
+    # 12 and 20 bodies (22 and 38 joints): the one-workgroup-per-island kernel with its
+    # system in HBM, an anchor last in each island
+    scene = scenes.anchored_chains(n_chains=4, m=m, seed=m)
+    kin = scene.mass == 0.0
+    print(f"anchored chains m={m}", flush=True)
+    dev = DeviceWorld(scene, tol2=1e-20)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    s0 = dev.state()
+    dev.step(1, 1e-2, 1.0)
+    orc.step(1, 1e-2, 1.0)
+    a, b = dev.state(), orc.state()
+    assert rel(a["pose"], b["pose"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-12
+    dev.step(9, 1e-2, 1.0)
+    orc.step(9, 1e-2, 1.0)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-9
+    assert rel(a["momentum"], b["momentum"]) < 1e-9
+    np.testing.assert_array_equal(a["momentum"][kin], s0["momentum"][kin])
+    np.testing.assert_allclose(a["position"][kin], s0["position"][kin], rtol=0, atol=1e-12)
+    assert dev.fatals() == 0

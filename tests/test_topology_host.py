@@ -169,13 +169,14 @@ LMAXM, LMAXJ = 48, 128
 LMAXNBR, LMAXT = LMAXM + 2 * LMAXJ, 6 * LMAXM // 16
 
 
-def build_large(L, m, ja, jb):
+def build_large(L, m, ja, jb, md=None):
     lay = np.zeros(9, dtype=np.int64)
     L.tp_large_layout(lay.ctypes.data_as(C.POINTER(C.c_longlong)))
     buf = np.zeros(int(lay[8]), dtype=np.uint8)
     arr = [np.ascontiguousarray(x, dtype=np.int32) for x in (ja, jb)]
     ip = C.POINTER(C.c_int)
-    rc = L.tp_build_large(m, len(ja), *[x.ctypes.data_as(ip) for x in arr], buf.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    rc = L.tp_build_large_md(m, m if md is None else md, len(ja), *[x.ctypes.data_as(ip) for x in arr],
+                             buf.ctypes.data_as(C.POINTER(C.c_ubyte)))
     fields = [("nbr_start", "<u2", LMAXM + 1), ("nbr", "u1", LMAXNBR), ("nbr_row", "u1", LMAXNBR),
               ("term_start", "<u2", LMAXNBR + 1), ("terms", "<u2", 4 * LMAXJ), ("inc_start", "<u2", LMAXM + 1),
               ("inc", "<u2", 2 * LMAXJ), ("kmask", "<u4", 3 * LMAXT)]
@@ -312,5 +313,38 @@ def test_kinematic_bodies_have_no_rows_or_columns(tp, m, k):
         ends = (ja[j], jb[j])
         assert ends[re] < md and ends[ce] < md
     # incidence: every (joint, end) once, kinematic ends included
+    inc = sorted(int(c) for c in t["inc"][:t["inc_start"][m]])
+    assert inc == sorted([j << 1 for j in range(len(ja))] + [(j << 1) | 1 for j in range(len(ja))])
+
+
+@pytest.mark.parametrize("m,k", [(12, 1), (20, 3), (48, 1)])
+def test_large_record_kinematic_bodies_have_no_rows_or_columns(tp, m, k):
+    # the mechanism-sized record with the last k bodies kinematic has the blocks of the
+    # dynamic bodies alone (joints among them first, the joints to the kinematic bodies
+    # last); a joint to a kinematic body adds one term, its dynamic end's own, to that
+    # end's diagonal block; the incidence lists every joint at both ends
+    md = m - k
+    ja, jb = list(range(md - 1)), list(range(1, md))
+    ja += ja[::2]; jb += jb[::2]   # a second joint on every other link
+    dyn = (list(ja), list(jb))
+    for q in range(k):
+        ja.append(md + q); jb.append(q % md)
+    rc, t = build_large(tp, m, ja, jb, md=md)
+    rc0, t0 = build_large(tp, md, *dyn)
+    assert rc == 0 and rc0 == 0
+    nnz = int(t0["nbr_start"][md])
+    assert (t["nbr_start"][md:m + 1] == nnz).all()
+    np.testing.assert_array_equal(t["nbr_start"][:md + 1], t0["nbr_start"][:md + 1])
+    for f in ("nbr", "nbr_row"):
+        np.testing.assert_array_equal(t[f][:nnz], t0[f][:nnz])
+    np.testing.assert_array_equal(t["kmask"], t0["kmask"])
+    for q in range(nnz):
+        got = [int(c) for c in t["terms"][t["term_start"][q]:t["term_start"][q + 1]]]
+        want = [int(c) for c in t0["terms"][t0["term_start"][q]:t0["term_start"][q + 1]]]
+        bi, bj = int(t["nbr_row"][q]), int(t["nbr"][q])
+        for j in range(len(dyn[0]), len(ja)):   # joint j: end a kinematic, end b dynamic
+            if bi == bj == jb[j]:
+                want.append((j << 2) | (1 << 1) | 1)
+        assert got == want, (q, got, want)
     inc = sorted(int(c) for c in t["inc"][:t["inc_start"][m]])
     assert inc == sorted([j << 1 for j in range(len(ja))] + [(j << 1) | 1 for j in range(len(ja))])

@@ -84,8 +84,8 @@ def test_kinematic_bodies_first_are_refused():
     assert e.value.code == 6 and "kinematic bodies go last" in str(e.value)
 
 
-def test_kinematic_body_in_a_mechanism_sized_island_is_unsupported():
+def test_kinematic_body_in_a_mechanism_sized_island_passes_validation():
     s = scenes.anchored_chains(n_chains=1, m=12, seed=1)
     with pytest.raises(_lib.ExgError) as e:
         DeviceWorld(s)
-    assert e.value.code == 7 and "kinematic" in str(e.value)
+    assert e.value.code == 5, str(e.value)
