@@ -154,6 +154,8 @@ struct exg_world {
     DevBuf<exg::GravEntry> grav_entries;
     DevBuf<exg::GravTile> grav_tiles;
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
+    bool grav_island_reduce = true;   // the islands add the gravity partial sums (where eligible)
+    bool island_reduce_ok = false;    // every island on-chip and every tile within the islands' chunk cap
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;              // island-steps that ended with a non-finite state
     DevBuf<int> requeue;            // single-body islands whose step subdivides (one per such island)
@@ -286,6 +288,9 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
         W_TRY(w->grav_tiles.alloc(tiles.size()));
         W_TRY(w->grav_tiles.upload(tiles.data(), tiles.size()));
         W_TRY(w->grav_partial.alloc(entries.size() * exg::GRAV_PARTIAL_PER_ENTRY));
+        bool chunks_ok = true;
+        for (const auto& tl : tiles) chunks_ok &= tl.ne <= exg::GRAV_ISLAND_REDUCE_MAX_CHUNKS;
+        w->island_reduce_ok = chunks_ok;   // and every island on-chip (checked with the classes below)
         W_TRY(w->grav_stage.alloc(B * exg::GRAV_STAGE_BYTES / sizeof(double)));
     }
     W_TRY(w->island_start.alloc(I + 1));
@@ -306,6 +311,8 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
             W_TRY(w->class_ids[c].alloc(ids[c].size()));
             W_TRY(hipMemcpy(w->class_ids[c].p, ids[c].data(), ids[c].size() * sizeof(int), hipMemcpyHostToDevice));
         }
+        if (w->class_n[exg::ISLAND_CLASS_M1] > 0 || w->class_n[exg::ISLAND_CLASS_LARGE] > 0)
+            w->island_reduce_ok = false;   // free bodies and mechanism-sized islands read ext
         if (w->class_n[exg::ISLAND_CLASS_M1] > 0) {
             W_TRY(w->requeue.alloc(w->class_n[exg::ISLAND_CLASS_M1]));
             W_TRY(w->requeue_ctl.alloc(2));
@@ -420,6 +427,13 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian) {
     return EXG_OK;
 }
 
+int exg_world_set_gravity_island_reduce(exg_world* w, int on) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_gravity_island_reduce: null world");
+    w->grav_island_reduce = on != 0;
+    w->settings_epoch++;
+    return EXG_OK;
+}
+
 int exg_world_set_ns_engine(exg_world* w, int engine) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_ns_engine: null world");
     if (engine != EXG_NS_MFMA && engine != EXG_NS_VALU_ORDERED)
@@ -446,12 +460,16 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     // timing events are recorded by the dispatches themselves (no marker packets)
     hipEvent_t* ev = w->step_ev;
     hipEvent_t g0 = ev ? ev[0] : nullptr, g1 = ev ? ev[1] : nullptr;
-    if (w->gravity) {
+    // the islands add the f64 partial sums themselves (the reduce kernel is skipped):
+    // every island in a 4- or 8-body class, the Newton integrator, f64 pairs
+    const bool island_reduce = w->gravity && w->grav_island_reduce && w->island_reduce_ok &&
+                               w->integrator == EXG_NEWTON && w->grav_prec == EXG_PREC_F64 && w->grav_entries.n > 0;
+    if (w->gravity)
         exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p, (int)w->grav_tiles.n,
-                            w->grav_prec, w->grav_partial.p, w->ext.p, w->grav_stage.p, w->B, w->stream, g0, g1);
-    } else {
+                            w->grav_prec, w->grav_partial.p, w->ext.p, w->grav_stage.p, w->B, w->stream, g0, g1,
+                            !island_reduce);
+    else
         exg::launch_zero_wrench(w->ext.p, 0, w->B, w->stream, g0, g1);
-    }
     HIP_TRY(hipGetLastError());
     if (ev && !(w->gravity ? w->grav_tiles.n > 0 : w->B > 0)) {  // nothing launched: mark the (empty) phase
         HIP_TRY(hipEventRecord(g0, w->stream));
@@ -494,6 +512,9 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.ns_engine = w->ns_engine;
     a.jac_f32 = w->jac_prec == EXG_PREC_F32_COMPENSATED;
     a.n_islands = w->I;
+    a.grav_tiles = island_reduce ? w->grav_tiles.p : nullptr;
+    a.grav_partial = w->grav_partial.p;
+    a.grav_charges = front;
     hipEvent_t i0 = ev ? ev[2] : nullptr, i1 = ev ? ev[3] : nullptr;
     if (w->integrator == EXG_NEWTON) {
         int* ids[exg::ISLAND_CLASSES];

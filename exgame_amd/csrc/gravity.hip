@@ -41,15 +41,6 @@ static_assert(GRAV_TILE == 64, "one target per lane of a wave");
 constexpr int GRAV_WAVES = 4;     // source slices per workgroup
 constexpr int GRAV_MIN_CHUNK = 64;
 
-// The hardware estimate (≈ 2^-24 relative) refined by one third-order step,
-// y = y0 + y0·e·(1/2 + 3/8·e) with e = 1 − x·y0²: within 2 ulp of 1/sqrt(x) over
-// 1e-40 … 2e30 (the bound of two Newton steps, in five operations instead of eight).
-__device__ inline double rsqrt_refined(double x) {
-    const double y0 = __builtin_amdgcn_rsq(x);
-    const double e = fma(-(x * y0), y0, 1.0);
-    const double c = fma(0.375, e, 0.5);
-    return fma(y0 * e, c, y0);
-}
 
 // Wave-local staging: a wave's LDS rows are written and read by that wave only.
 __device__ inline void wave_lds_sync() {
@@ -375,16 +366,7 @@ __device__ inline void sum_partials(const GravTile& tl, const double* __restrict
 // Scale by the target's charge / G and write the wrench with the moment local × F.
 __device__ inline void write_wrench(const double* __restrict__ ch, const GravTile& tl, int i, double f[3],
                                     double* __restrict__ ext) {
-    double* o = ext + (size_t)i * 6;
-    const double a = ch[(size_t)i * 8 + 6] * tl.g_inv;
-    for (int k = 0; k < 3; ++k) f[k] *= a;
-    const double lx = ch[(size_t)i * 8 + 3], ly = ch[(size_t)i * 8 + 4], lz = ch[(size_t)i * 8 + 5];
-    o[0] = f[0];
-    o[1] = f[1];
-    o[2] = f[2];
-    o[3] = ly * f[2] - lz * f[1];
-    o[4] = lz * f[0] - lx * f[2];
-    o[5] = lx * f[1] - ly * f[0];
+    gravity_wrench(ch + (size_t)i * 8, tl.g_inv, f, ext + (size_t)i * 6);
 }
 
 __global__ __launch_bounds__(GRAV_TILE) void gravity_reduce_kernel(const double* __restrict__ ch,
@@ -517,7 +499,7 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
 
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
                     int n_tiles, int precision, double* partial, double* ext, void* stage, int n_bodies,
-                    hipStream_t stream, hipEvent_t start, hipEvent_t stop) {
+                    hipStream_t stream, hipEvent_t start, hipEvent_t stop, bool reduce) {
     if (n_tiles <= 0) return;
     // tiles without entries (worlds without gravity) still get their zero wrench from
     // the reduce, also when no world of the launch has gravity
@@ -531,9 +513,11 @@ void launch_gravity(const double* charges, const GravEntry* entries, int n_entri
                  n_bodies, (float4*)stage);
         launch_k(gravity_partial_f32c_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, nullptr, nullptr, charges,
                  entries, (const float4*)stage, n_bodies, partial);
-    } else
-        launch_k(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start, nullptr, charges,
-                 entries, partial);
+    } else {
+        launch_k(gravity_partial_f64_kernel, dim3(n_entries), dim3(64 * GRAV_WAVES), stream, start,
+                 reduce ? nullptr : stop, charges, entries, partial);
+        if (!reduce) return;   // the islands add the partial sums
+    }
     if (precision == 1)
         launch_k(gravity_reduce_f32c_kernel, dim3(n_tiles), dim3(GRAV_TILE * GRAV_WAVES), stream, nullptr, stop, charges,
                  tiles, (const double*)partial, ext);

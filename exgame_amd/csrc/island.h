@@ -84,6 +84,8 @@ constexpr int ISLAND_THREADS = 256;
 constexpr int ISLAND_STATS = 8;
 constexpr int STAT_FATAL = 6, STAT_NONFINITE_SPANS = 7;
 
+struct GravTile;
+
 // One implicit step of every island (one workgroup per island).
 struct IslandArgs {
     double* pose;             // B×8 motors, island frame
@@ -120,7 +122,25 @@ struct IslandArgs {
     int* requeue;             // single-body islands whose step subdivides, for the requeue kernel (capacity:
                               // the single-body islands of the launch); null: the one-lane kernel solves all
     unsigned* requeue_ctl;    // [0] the list's length, [1] requeue workgroups done; both 0 between launches
+    // gravity reduced by the islands (4- and 8-body classes, f64 pairs): with grav_tiles
+    // set, each island adds its bodies' partial sums (grav_partial, from the gravity
+    // pass's partial kernel) and forms their wrenches from the published charges
+    // (grav_charges) itself, and ext is not read; null: ext holds the wrenches
+    const GravTile* grav_tiles;
+    const double* grav_partial;
+    const double* grav_charges;
 };
+
+// The f64 pair law's 1/sqrt: the hardware estimate (≈ 2^-24 relative) refined by one
+// third-order step, y = y0 + y0·e·(1/2 + 3/8·e) with e = 1 − x·y0²: within 2 ulp of
+// 1/sqrt(x) over 1e-40 … 2e30 (the bound of two Newton steps, in five operations
+// instead of eight). Used by the gravity pass and by the islands' fused gravity.
+__device__ inline double rsqrt_refined(double x) {
+    const double y0 = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y0), y0, 1.0);
+    const double c = fma(0.375, e, 0.5);
+    return fma(y0 * e, c, y0);
+}
 
 // Mark-and-continue fatal report of one island-step (the state is written as it is):
 // the island's STAT_FATAL counter and the launch-wide counter.
@@ -207,6 +227,22 @@ struct GravTile {
 };
 constexpr int GRAV_TILE = 64;                     // targets per tile
 constexpr int GRAV_PARTIAL_PER_ENTRY = 3 * GRAV_TILE;   // doubles of partial sums per entry
+constexpr int GRAV_ISLAND_REDUCE_MAX_CHUNKS = 64;        // chunks per tile the islands' reduction holds
+// A target's wrench from its summed pair force f (Σ_j G·m_j·r⁻³·(c_j − c_i)): scaled by
+// its charge G·m over G, with the moment about its island origin from its lever arm
+// (charge record [x y z | lx ly lz | G·m | size]). Shared by the reduce kernel and the
+// islands' reduction so that both give the same bits.
+__device__ inline void gravity_wrench(const double* own, double g_inv, double f[3], double* out) {
+    const double a = own[6] * g_inv;
+    for (int k = 0; k < 3; ++k) f[k] *= a;
+    const double lx = own[3], ly = own[4], lz = own[5];
+    out[0] = f[0];
+    out[1] = f[1];
+    out[2] = f[2];
+    out[3] = ly * f[2] - lz * f[1];
+    out[4] = lz * f[0] - lx * f[2];
+    out[5] = lx * f[1] - ly * f[0];
+}
 void gravity_add_world(std::vector<GravEntry>& entries, std::vector<GravTile>& tiles, int b0, int n, double G,
                        int compute_units);
 // The chunk length gravity_add_world uses for a world of n bodies (a pure function).
@@ -218,9 +254,11 @@ void gravity_add_world_chunked(std::vector<GravEntry>& entries, std::vector<Grav
 // `stage` holds GRAV_STAGE_BYTES per body for bodies [0, n_bodies) (every body an
 // entry names): the per-source records the f32 kernel reads.
 constexpr int GRAV_STAGE_BYTES = 64;
+// With reduce = false (f64 pairs, entries present) only the partial kernel runs: the
+// islands add the partial sums themselves (IslandArgs::grav_tiles).
 void launch_gravity(const double* charges, const GravEntry* entries, int n_entries, const GravTile* tiles,
                     int n_tiles, int precision, double* partial, double* ext, void* stage, int n_bodies,
-                    hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                    hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr, bool reduce = true);
 void launch_zero_wrench(double* ext, int b0, int n, hipStream_t stream, hipEvent_t start = nullptr,
                         hipEvent_t stop = nullptr);
 

@@ -1041,6 +1041,38 @@ struct IslandSolver {
     }
 };
 
+// ── the island's gravity wrenches from the gravity pass's partial sums ──
+// In place of the gravity pass's reduce kernel, in two halves around the load phase's
+// first barrier: thread 3i + k < 3m adds body i's axis-k partial sums over the tile's
+// chunks in ascending order from 0 (loads issued four chunks at a time, beside the
+// island's other loads) into s.ext[i][k]; after the barrier thread i < m forms the
+// wrench with gravity_wrench. These are the reduce kernel's sums and arithmetic, so the
+// wrenches are the same bits. Tiles are GRAV_TILE consecutive targets from body 0.
+__device__ __forceinline__ void island_gravity_sum(const IslandArgs& a, int b0, int m, double (*ext)[6]) {
+    const int t = threadIdx.x;
+    if (t >= 3 * m) return;
+    const int i = t / 3, k = t - 3 * i, gi = b0 + i;
+    const GravTile& tl = a.grav_tiles[gi / GRAV_TILE];
+    const double* __restrict__ p = a.grav_partial + (size_t)tl.e0 * GRAV_PARTIAL_PER_ENTRY + k * GRAV_TILE + (gi - tl.tgt0);
+    double f = 0.0;
+    int c = 0;
+    for (; c + 4 <= tl.ne; c += 4) {
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = p[(size_t)(c + u) * GRAV_PARTIAL_PER_ENTRY];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) f += v[u];
+    }
+    for (; c < tl.ne; ++c) f += p[(size_t)c * GRAV_PARTIAL_PER_ENTRY];
+    ext[i][k] = f;
+}
+__device__ __forceinline__ void island_gravity_wrench(const IslandArgs& a, int b0, int m, double (*ext)[6]) {
+    const int t = threadIdx.x;
+    if (t >= m) return;
+    double f[3] = {ext[t][0], ext[t][1], ext[t][2]};
+    gravity_wrench(a.grav_charges + (size_t)(b0 + t) * 8, a.grav_tiles[(b0 + t) / GRAV_TILE].g_inv, f, ext[t]);
+}
+
 template <int MAXM, int MAXJ, int NTH, bool JF, bool FLAT = false>
 __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MAXM, MAXJ, NTH, JF>& s) {
     if ((int)blockIdx.x >= a.n_islands) return;
@@ -1049,11 +1081,13 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     const int j0 = a.island_jstart[isl], J = a.island_jend[isl] - j0;
     if (m > MAXM || J > MAXJ || m <= 0) return;  // the host routes islands by size; never taken
 
+    const bool island_reduce = a.grav_tiles != nullptr;
+    if (island_reduce) island_gravity_sum(a, b0, m, s.ext);
     // ── load the island ──
     for (int t = threadIdx.x; t < m * 8; t += NTH) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
     for (int t = threadIdx.x; t < m * 6; t += NTH) {
         s.mom[t / 6][t % 6] = a.mom[(size_t)(b0 + t / 6) * 6 + t % 6];
-        s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
+        if (!island_reduce) s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
         s.dv[t / 6][t % 6] = 0.0;
     }
     for (int t = threadIdx.x; t < m * 9; t += NTH) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];
@@ -1077,6 +1111,7 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
         s.u.T[e] = 0.0;
     }
     __syncthreads();
+    if (island_reduce) island_gravity_wrench(a, b0, m, s.ext);   // published by the barriers below
     for (int e = threadIdx.x; e < n * n; e += NTH) s.X[(e / n) * LD + e % n] = a.xacc[xoff + e];
     __syncthreads();
     // ── topology tables (built on the host from the joint graph, one record per island) ──

@@ -35,7 +35,8 @@ JOINT_MAPPINGS = {"split": 0, "flat": 1}
 class DeviceWorld:
     def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
-                 jacobian_precision: str = "f64", graph: bool = False, joint_mapping: str = "split"):
+                 jacobian_precision: str = "f64", graph: bool = False, joint_mapping: str = "split",
+                 gravity_island_reduce: bool = True):
         L = _lib.lib()
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
@@ -70,6 +71,8 @@ class DeviceWorld:
         _lib.check(L.exg_world_set_precision(self.h, PRECISIONS[gravity_precision], PRECISIONS[jacobian_precision]))
         _lib.check(L.exg_world_set_graph(self.h, int(bool(graph))))
         _lib.check(L.exg_world_set_joint_mapping(self.h, JOINT_MAPPINGS[joint_mapping]))
+        if not gravity_island_reduce:
+            _lib.check(L.exg_world_set_gravity_island_reduce(self.h, 0))
 
     def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
         if isinstance(dt, Epoch):

@@ -284,6 +284,15 @@ int exg_gravity_accumulate_noalloc(int n, const double* charges, double G, int p
  * and the state update stay f64, so a converged step meets the same f64 tolerance. */
 int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
 
+/* Who adds the gravity pass's partial sums (f64 pairs): with on = 1 (the default) the
+ * island kernels add their own bodies' partial sums in the reduce kernel's order and
+ * arithmetic (the same bits), and the reduce kernel is not launched; with on = 0 the
+ * reduce kernel runs. Takes effect for worlds whose islands all have 2 to 8 bodies,
+ * stepped by the Newton integrator with f64 gravity; the others run the reduce kernel
+ * either way. No counterpart in the reference (a schedule of GravityPropagator's
+ * accumulate, crates/newton/src/gravity.rs:530-601). */
+int exg_world_set_gravity_island_reduce(exg_world* w, int on);
+
 /* ───────────── arenas: many independent worlds stepped together ─────────────
  * The batched / served form of a Driver epoch stepping several mechanisms
  * (crates/newton/src/driver.rs; accelerator/mod.rs:161-277, 613-630 flush their rows
