@@ -41,19 +41,33 @@ class DeviceWorld:
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
         self.island_start = np.ascontiguousarray(s.island_start, dtype=np.int32)
+        # the library takes an island's kinematic bodies (mass 0) after its dynamic ones:
+        # bodies are uploaded in that order (dynamic bodies keep their relative order, so
+        # the gravity pass's slot rule between them is unchanged) and handed back in the
+        # scene's; _order[p] is the scene index of device body p, _back its inverse
+        kin = np.asarray(s.mass) == 0.0
+        order = np.concatenate([np.concatenate([np.flatnonzero(~kin[a:b]) + a, np.flatnonzero(kin[a:b]) + a])
+                                for a, b in zip(self.island_start[:-1], self.island_start[1:])]).astype(np.int64)
+        self._order = None if np.array_equal(order, np.arange(self.nb)) else order
+        self._back = None
+        if self._order is not None:
+            self._back = np.empty_like(order)
+            self._back[order] = np.arange(self.nb)
+        body = (lambda x: x) if self._order is None else (lambda x: np.asarray(x)[self._order])
+        ends = (lambda x: x) if self._order is None else (lambda x: self._back[np.asarray(x)])
         arrs = dict(
-            pose=np.ascontiguousarray(s.pose, dtype=np.float64),
-            mom=np.ascontiguousarray(s.momentum, dtype=np.float64),
-            mass=np.ascontiguousarray(s.mass, dtype=np.float64),
-            tens=np.ascontiguousarray(s.inertia, dtype=np.float64),
-            size=np.ascontiguousarray(s.size, dtype=np.float64),
+            pose=np.ascontiguousarray(body(s.pose), dtype=np.float64),
+            mom=np.ascontiguousarray(body(s.momentum), dtype=np.float64),
+            mass=np.ascontiguousarray(body(s.mass), dtype=np.float64),
+            tens=np.ascontiguousarray(body(s.inertia), dtype=np.float64),
+            size=np.ascontiguousarray(body(s.size), dtype=np.float64),
             origin=np.ascontiguousarray(s.origin, dtype=np.float64),
             jp=np.ascontiguousarray(s.joint_params, dtype=np.float64).reshape(-1),
         )
         ints = dict(
             jk=np.ascontiguousarray(s.joint_kind, dtype=np.int32),
-            ja=np.ascontiguousarray(s.joint_a, dtype=np.int32),
-            jb=np.ascontiguousarray(s.joint_b, dtype=np.int32),
+            ja=np.ascontiguousarray(ends(s.joint_a), dtype=np.int32),
+            jb=np.ascontiguousarray(ends(s.joint_b), dtype=np.int32),
         )
         if self.nj == 0:
             arrs["jp"] = np.zeros(10)
@@ -87,6 +101,7 @@ class DeviceWorld:
         mom = np.zeros((self.nb, 6))
         origin = np.zeros((self.ni, 3))
         _lib.check(_lib.lib().exg_world_download(self.h, _lib.fptr(pose), _lib.fptr(mom), _lib.fptr(origin)))
+        pose, mom = self._scene_order(pose), self._scene_order(mom)
         isl = np.repeat(np.arange(self.ni), np.diff(self.island_start))
         position = pga.position(pose) + origin[isl]
         return {"pose": pose, "momentum": mom, "origin": origin, "position": position}
@@ -94,12 +109,16 @@ class DeviceWorld:
     def charges(self) -> np.ndarray:
         out = np.zeros((self.nb, 8))
         _lib.check(_lib.lib().exg_world_charges(self.h, _lib.fptr(out)))
-        return out
+        return self._scene_order(out)
+
+    def _scene_order(self, x: np.ndarray) -> np.ndarray:
+        """Per-body rows from the device's order back to the scene's."""
+        return x if self._back is None else x[self._back]
 
     def external(self) -> np.ndarray:
         out = np.zeros((self.nb, 6))
         _lib.check(_lib.lib().exg_world_external(self.h, _lib.fptr(out)))
-        return out
+        return self._scene_order(out)
 
     def stats(self) -> np.ndarray:
         out = np.zeros((self.ni, 8), dtype=np.int32)

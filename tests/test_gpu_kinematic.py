@@ -156,3 +156,19 @@ def test_anchored_long_chains_on_the_mechanism_sized_kernel(m):
     np.testing.assert_array_equal(a["momentum"][kin], s0["momentum"][kin])
     np.testing.assert_allclose(a["position"][kin], s0["position"][kin], rtol=0, atol=1e-12)
     assert dev.fatals() == 0
+
+
+@pytest.mark.parametrize("m", [4, 8])
+def test_kinematic_bodies_anywhere_in_their_island(m):
+    # the anchor first in each island: DeviceWorld uploads the dynamic bodies first and
+    # hands the state back in the scene's order; against the oracle stepping the scene
+    # as it is (the reference's order map, newton.rs:806-830)
+    scene = scenes.anchored_chains(n_chains=8, m=m, seed=7, kinematic_first=True)
+    kin = scene.mass == 0.0
+    assert kin[0] and kin[m]
+    dev, orc = run_pair(scene, 3)
+    a, b = dev.state(), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-11
+    assert rel(dev.charges(), orc.front()) < 1e-12
+    np.testing.assert_array_equal(a["momentum"][kin], scene.momentum[kin])

@@ -20,7 +20,7 @@ def bad(scene, **fields):
 @pytest.mark.parametrize("field,idx,val,code", [
     ("mass", 3, -1.0, 6),          # negative mass: invalid
     ("mass", 2, np.nan, 6),
-    ("mass", 1, 0.0, 6),           # a kinematic body (mass 0) before dynamic ones in its island
+    ("mass", 1, np.inf, 6),
     ("inertia", 9 * 2 + 4, -0.5, 6),   # not positive definite
     ("inertia", 0, np.inf, 6),
     ("pose", 8 * 1 + 5, np.nan, 6),
@@ -77,11 +77,31 @@ def test_kinematic_bodies_last_in_their_islands_pass_validation():
     assert e.value.code == 5, str(e.value)
 
 
-def test_kinematic_bodies_first_are_refused():
+def test_kinematic_bodies_first_are_refused_by_the_c_abi():
+    # the C ABI takes an island's kinematic bodies last (DeviceWorld reorders for it)
+    s = scenes.anchored_chains(n_chains=2, m=4, seed=1, kinematic_first=True)
+    L = _lib.lib()
+    i32 = lambda x: np.ascontiguousarray(x, dtype=np.int32)
+    f64 = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+    arrs = [f64(s.pose), f64(s.momentum), f64(s.mass), f64(s.inertia), f64(s.size)]
+    ist, org = i32(s.island_start), f64(s.origin)
+    jk, ja, jb, jp = i32(s.joint_kind), i32(s.joint_a), i32(s.joint_b), f64(s.joint_params)
+    import ctypes as C
+    h = C.c_void_p()
+    rc = L.exg_world_create(s.n_bodies, *[_lib.fptr(x) for x in arrs], s.n_islands, _lib.iptr(ist), _lib.fptr(org),
+                            s.n_joints, _lib.iptr(jk), _lib.iptr(ja), _lib.iptr(jb), _lib.fptr(jp), 1.0, 1,
+                            C.byref(h))
+    assert rc == 6
+    assert "kinematic bodies go last" in L.exg_last_error().decode()
+
+
+def test_device_world_takes_kinematic_bodies_anywhere():
+    # DeviceWorld uploads each island's dynamic bodies first: validation passes (and
+    # creation goes on to the absent device)
     s = scenes.anchored_chains(n_chains=2, m=4, seed=1, kinematic_first=True)
     with pytest.raises(_lib.ExgError) as e:
         DeviceWorld(s)
-    assert e.value.code == 6 and "kinematic bodies go last" in str(e.value)
+    assert e.value.code == 5, str(e.value)
 
 
 def test_kinematic_body_in_a_mechanism_sized_island_passes_validation():
