@@ -115,18 +115,22 @@ class Arena:
 
     def admit(self, scene, G: float = 1.0, gravity: bool = True):
         """World id, or None when the arena has no room for it now."""
+        from .world import kinematic_last_order
         s = scene
-        pose = np.ascontiguousarray(s.pose, dtype=np.float64)
-        mom = np.ascontiguousarray(s.momentum, dtype=np.float64)
-        mass = np.ascontiguousarray(s.mass, dtype=np.float64)
-        tens = np.ascontiguousarray(s.inertia, dtype=np.float64)
-        size = np.ascontiguousarray(s.size, dtype=np.float64)
-        origin = np.ascontiguousarray(s.origin, dtype=np.float64)
         ist = np.ascontiguousarray(s.island_start, dtype=np.int32)
+        order, back = kinematic_last_order(s.mass, ist)   # kinematic bodies last in each island
+        body = (lambda x: x) if order is None else (lambda x: np.asarray(x)[order])
+        ends = (lambda x: x) if order is None else (lambda x: back[np.asarray(x)])
+        pose = np.ascontiguousarray(body(s.pose), dtype=np.float64)
+        mom = np.ascontiguousarray(body(s.momentum), dtype=np.float64)
+        mass = np.ascontiguousarray(body(s.mass), dtype=np.float64)
+        tens = np.ascontiguousarray(body(s.inertia), dtype=np.float64)
+        size = np.ascontiguousarray(body(s.size), dtype=np.float64)
+        origin = np.ascontiguousarray(s.origin, dtype=np.float64)
         if s.n_joints:
             jk = np.ascontiguousarray(s.joint_kind, dtype=np.int32)
-            ja = np.ascontiguousarray(s.joint_a, dtype=np.int32)
-            jb = np.ascontiguousarray(s.joint_b, dtype=np.int32)
+            ja = np.ascontiguousarray(ends(s.joint_a), dtype=np.int32)
+            jb = np.ascontiguousarray(ends(s.joint_b), dtype=np.int32)
             jp = np.ascontiguousarray(s.joint_params, dtype=np.float64).reshape(-1)
         else:
             jk = ja = jb = np.zeros(1, dtype=np.int32)
@@ -139,7 +143,7 @@ class Arena:
         if rc == 8:
             return None
         _lib.check(rc)
-        self.shapes[wid.value] = (s.n_bodies, s.n_islands, np.diff(ist).astype(np.int64))
+        self.shapes[wid.value] = (s.n_bodies, s.n_islands, np.diff(ist).astype(np.int64), back)
         return wid.value
 
     def retire(self, wid: int) -> None:
@@ -172,16 +176,18 @@ class Arena:
         return v.value
 
     def state(self, wid: int) -> dict:
-        nb, ni, sizes = self.shapes[wid]
+        nb, ni, sizes, back = self.shapes[wid]
         pose = np.zeros((nb, 8))
         mom = np.zeros((nb, 6))
         origin = np.zeros((ni, 3))
         _lib.check(self.L.exg_arena_download(self.h, int(wid), _lib.fptr(pose), _lib.fptr(mom), _lib.fptr(origin)))
+        if back is not None:   # back to the scene's body order
+            pose, mom = pose[back], mom[back]
         return {"pose": pose, "momentum": mom, "origin": origin,
                 "position": pga.position(pose) + np.repeat(origin, sizes, axis=0)}
 
     def stats(self, wid: int) -> np.ndarray:
-        nb, ni, _ = self.shapes[wid]
+        nb, ni, _, _ = self.shapes[wid]
         out = np.zeros((ni, 8), dtype=np.int32)
         _lib.check(self.L.exg_arena_world_stats(self.h, int(wid), out.ctypes.data_as(C.POINTER(C.c_int32))))
         return out

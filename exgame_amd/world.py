@@ -21,6 +21,24 @@ NS_ENGINES = {"mfma": 0, "valu_ordered": 1}
 PRECISIONS = {"f64": 0, "f32c": 1}
 
 
+def kinematic_last_order(mass, island_start):
+    """The library takes an island's kinematic bodies (mass 0) after its dynamic ones.
+    Returns (order, back): order[p] is the scene index of the body uploaded at p (each
+    island's dynamic bodies in their order, then its kinematic ones), back its inverse;
+    (None, None) when the scene already has that order. Dynamic bodies keep their
+    relative order, so the gravity pass's higher-slot rule between them is unchanged
+    (pairs with a kinematic body carry no force)."""
+    kin = np.asarray(mass) == 0.0
+    st = np.asarray(island_start)
+    order = np.concatenate([np.concatenate([np.flatnonzero(~kin[a:b]) + a, np.flatnonzero(kin[a:b]) + a])
+                            for a, b in zip(st[:-1], st[1:])]).astype(np.int64)
+    if np.array_equal(order, np.arange(len(kin))):
+        return None, None
+    back = np.empty_like(order)
+    back[order] = np.arange(len(kin))
+    return order, back
+
+
 @dataclass(frozen=True)
 class Epoch:
     """dt and time warp of one step (crates/aristotle/src/epoch.rs:238-248)."""
@@ -41,18 +59,7 @@ class DeviceWorld:
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
         self.island_start = np.ascontiguousarray(s.island_start, dtype=np.int32)
-        # the library takes an island's kinematic bodies (mass 0) after its dynamic ones:
-        # bodies are uploaded in that order (dynamic bodies keep their relative order, so
-        # the gravity pass's slot rule between them is unchanged) and handed back in the
-        # scene's; _order[p] is the scene index of device body p, _back its inverse
-        kin = np.asarray(s.mass) == 0.0
-        order = np.concatenate([np.concatenate([np.flatnonzero(~kin[a:b]) + a, np.flatnonzero(kin[a:b]) + a])
-                                for a, b in zip(self.island_start[:-1], self.island_start[1:])]).astype(np.int64)
-        self._order = None if np.array_equal(order, np.arange(self.nb)) else order
-        self._back = None
-        if self._order is not None:
-            self._back = np.empty_like(order)
-            self._back[order] = np.arange(self.nb)
+        self._order, self._back = kinematic_last_order(s.mass, self.island_start)
         body = (lambda x: x) if self._order is None else (lambda x: np.asarray(x)[self._order])
         ends = (lambda x: x) if self._order is None else (lambda x: self._back[np.asarray(x)])
         arrs = dict(

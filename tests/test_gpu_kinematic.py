@@ -172,3 +172,16 @@ def test_kinematic_bodies_anywhere_in_their_island(m):
     assert rel(a["momentum"], b["momentum"]) < 1e-11
     assert rel(dev.charges(), orc.front()) < 1e-12
     np.testing.assert_array_equal(a["momentum"][kin], scene.momentum[kin])
+
+
+def test_arena_takes_kinematic_bodies_anywhere():
+    scene = scenes.anchored_chains(n_chains=8, m=8, seed=8, kinematic_first=True)
+    ar = Arena(bodies=1 << 15, islands=1 << 13, joints=1 << 14, inverses=1 << 22, tol2=1e-20)
+    ar.admit(scenes.mixed_world(200, seed=3))
+    wid = ar.admit(scene)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    ar.step(3, 1e-2, 1.0)
+    orc.step(3, 1e-2, 1.0)
+    a, b = ar.state(wid), orc.state()
+    assert rel(a["position"], b["position"]) < 1e-12
+    assert rel(a["momentum"], b["momentum"]) < 1e-11
