@@ -57,6 +57,7 @@ SIGNATURES = {
     "exg_gravity_accumulate_precision": [C.c_int, _dp, C.c_double, C.c_int, _dp],
     "exg_world_set_precision": [C.c_void_p, C.c_int, C.c_int],
     "exg_world_set_gravity_island_reduce": [C.c_void_p, C.c_int],
+    "exg_world_set_uniform_field": [C.c_void_p, C.c_double, C.c_double, C.c_double],
 }
 
 _lib = None

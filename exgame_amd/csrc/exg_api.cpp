@@ -155,6 +155,8 @@ struct exg_world {
     DevBuf<exg::GravTile> grav_tiles;
     int grav_prec = EXG_PREC_F64, jac_prec = EXG_PREC_F64;
     bool grav_island_reduce = true;   // the islands add the gravity partial sums (where eligible)
+    double field[3] = {0.0, 0.0, 0.0}; // UniformField acceleration
+    bool field_on = false;
     bool island_reduce_ok = false;    // every island on-chip and every tile within the islands' chunk cap
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;              // island-steps that ended with a non-finite state
@@ -427,6 +429,18 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian) {
     return EXG_OK;
 }
 
+int exg_world_set_uniform_field(exg_world* w, double ax, double ay, double az) {
+    if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_uniform_field: null world");
+    if (!std::isfinite(ax) || !std::isfinite(ay) || !std::isfinite(az))
+        return fail(EXG_ERR_INVALID, "exg_world_set_uniform_field: acceleration is not finite");
+    w->field[0] = ax;
+    w->field[1] = ay;
+    w->field[2] = az;
+    w->field_on = ax != 0.0 || ay != 0.0 || az != 0.0;
+    w->settings_epoch++;
+    return EXG_OK;
+}
+
 int exg_world_set_gravity_island_reduce(exg_world* w, int on) {
     if (!w) return fail(EXG_ERR_INVALID, "exg_world_set_gravity_island_reduce: null world");
     w->grav_island_reduce = on != 0;
@@ -512,6 +526,8 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     a.ns_engine = w->ns_engine;
     a.jac_f32 = w->jac_prec == EXG_PREC_F32_COMPENSATED;
     a.n_islands = w->I;
+    for (int k = 0; k < 3; ++k) a.field[k] = w->field[k];
+    a.field_on = w->field_on ? 1 : 0;
     a.grav_tiles = island_reduce ? w->grav_tiles.p : nullptr;
     a.grav_partial = w->grav_partial.p;
     a.grav_charges = front;

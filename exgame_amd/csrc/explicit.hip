@@ -77,7 +77,7 @@ __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs 
         for (int k = 0; k < 8; ++k) pose[i][k] = a.pose[(size_t)(b0 + i) * 8 + k];
         for (int k = 0; k < 6; ++k) {
             mom[i][k] = a.mom[(size_t)(b0 + i) * 6 + k];
-            total[i][k] = a.ext[(size_t)(b0 + i) * 6 + k];
+            total[i][k] = field_add(a, a.mass[b0 + i], k, a.ext[(size_t)(b0 + i) * 6 + k]);
         }
         world_velocity(pose[i], a.mass[b0 + i], a.tens + (size_t)(b0 + i) * 9, mom[i], wv[i]);
     }

@@ -129,7 +129,17 @@ struct IslandArgs {
     const GravTile* grav_tiles;
     const double* grav_partial;
     const double* grav_charges;
+    double field[3];          // UniformField acceleration (world frame), applied when field_on
+    int field_on;
 };
+
+// UniformField (crates/newton/src/field.rs:16-44): a constant world acceleration adds
+// F = m·a and a zero torque to a body's wrench after gravity (a kinematic body, mass 0,
+// gets nothing). Wrench component k of a body of mass m; no add at all when the field
+// is off, so that the wrenches keep their bits.
+__device__ inline double field_add(const IslandArgs& a, double m, int k, double w) {
+    return a.field_on ? w + (k < 3 ? a.field[k] * m : 0.0) : w;
+}
 
 // The f64 pair law's 1/sqrt: the hardware estimate (≈ 2^-24 relative) refined by one
 // third-order step, y = y0 + y0·e·(1/2 + 3/8·e) with e = 1 − x·y0²: within 2 ulp of

@@ -1071,6 +1071,7 @@ __device__ __forceinline__ void island_gravity_wrench(const IslandArgs& a, int b
     if (t >= m) return;
     double f[3] = {ext[t][0], ext[t][1], ext[t][2]};
     gravity_wrench(a.grav_charges + (size_t)(b0 + t) * 8, a.grav_tiles[(b0 + t) / GRAV_TILE].g_inv, f, ext[t]);
+    for (int k = 0; k < 6; ++k) ext[t][k] = field_add(a, a.mass[b0 + t], k, ext[t][k]);
 }
 
 template <int MAXM, int MAXJ, int NTH, bool JF, bool FLAT = false>
@@ -1087,7 +1088,8 @@ __device__ __forceinline__ void island_step(const IslandArgs& a, IslandShared<MA
     for (int t = threadIdx.x; t < m * 8; t += NTH) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
     for (int t = threadIdx.x; t < m * 6; t += NTH) {
         s.mom[t / 6][t % 6] = a.mom[(size_t)(b0 + t / 6) * 6 + t % 6];
-        if (!island_reduce) s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
+        if (!island_reduce)
+            s.ext[t / 6][t % 6] = field_add(a, a.mass[b0 + t / 6], t % 6, a.ext[(size_t)(b0 + t / 6) * 6 + t % 6]);
         s.dv[t / 6][t % 6] = 0.0;
     }
     for (int t = threadIdx.x; t < m * 9; t += NTH) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];

@@ -581,7 +581,7 @@ __global__ __launch_bounds__(NT) void island_step_kernel_large(IslandArgs a) {
     for (int t = threadIdx.x; t < m * 8; t += NT) s.pose[t / 8][t % 8] = a.pose[(size_t)(b0 + t / 8) * 8 + t % 8];
     for (int t = threadIdx.x; t < m * 6; t += NT) {
         s.mom[t / 6][t % 6] = a.mom[(size_t)(b0 + t / 6) * 6 + t % 6];
-        s.ext[t / 6][t % 6] = a.ext[(size_t)(b0 + t / 6) * 6 + t % 6];
+        s.ext[t / 6][t % 6] = field_add(a, a.mass[b0 + t / 6], t % 6, a.ext[(size_t)(b0 + t / 6) * 6 + t % 6]);
         s.dv[t / 6][t % 6] = 0.0;
     }
     for (int t = threadIdx.x; t < m * 9; t += NT) s.tens[t / 9][t % 9] = a.tens[(size_t)(b0 + t / 9) * 9 + t % 9];

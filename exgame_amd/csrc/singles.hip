@@ -582,7 +582,7 @@ __global__ __launch_bounds__(SB) void singles_step_kernel(IslandArgs a) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
         sg.mom[q] = a.mom[(size_t)b * 6 + q];
-        sg.ext[q] = a.ext[(size_t)b * 6 + q];
+        sg.ext[q] = field_add(a, a.mass[b], q, a.ext[(size_t)b * 6 + q]);
         sg.dv[q] = 0.0;
     }
 #pragma unroll
@@ -933,7 +933,7 @@ __global__ __launch_bounds__(SB8) void singles_requeue_kernel(IslandArgs a) {
         for (int t = 0; t < 8; ++t) sg.pose[t] = a.pose[(size_t)b * 8 + t];
         for (int t = 0; t < 6; ++t) {
             sg.mom[t] = a.mom[(size_t)b * 6 + t];
-            sg.ext[t] = a.ext[(size_t)b * 6 + t];
+            sg.ext[t] = field_add(a, a.mass[b], t, a.ext[(size_t)b * 6 + t]);
             sg.dv[t] = 0.0;
             sg.Ar[t] = 0.0;
             sg.Tr[t] = 0.0;

@@ -54,7 +54,7 @@ class DeviceWorld:
     def __init__(self, scene, G: float = 1.0, gravity: bool = True, integrator: str = "newton", tol2: float = 1e-8,
                  ns_warm: int = 2, ns_seed: int = 24, ns_engine: str = "mfma", gravity_precision: str = "f64",
                  jacobian_precision: str = "f64", graph: bool = False, joint_mapping: str = "split",
-                 gravity_island_reduce: bool = True):
+                 gravity_island_reduce: bool = True, uniform_field=None):
         L = _lib.lib()
         s = scene
         self.nb, self.ni, self.nj = s.n_bodies, s.n_islands, s.n_joints
@@ -94,6 +94,9 @@ class DeviceWorld:
         _lib.check(L.exg_world_set_joint_mapping(self.h, JOINT_MAPPINGS[joint_mapping]))
         if not gravity_island_reduce:
             _lib.check(L.exg_world_set_gravity_island_reduce(self.h, 0))
+        if uniform_field is not None:   # UniformField: F = m·a on every body (field.rs:16-44)
+            ax, ay, az = (float(v) for v in uniform_field)
+            _lib.check(L.exg_world_set_uniform_field(self.h, ax, ay, az))
 
     def step(self, n: int = 1, dt: float = 1e-2, warp: float = 1.0) -> None:
         if isinstance(dt, Epoch):

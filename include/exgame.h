@@ -293,6 +293,13 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
  * accumulate, crates/newton/src/gravity.rs:530-601). */
 int exg_world_set_gravity_island_reduce(exg_world* w, int on);
 
+/* A uniform force field on the world (UniformField, crates/newton/src/field.rs:16-44,
+ * a ForceField beside the N-body gravity): a constant world acceleration a adds
+ * F = m·a and a zero torque to every body's wrench after gravity, in every integrator;
+ * kinematic bodies (mass 0) get nothing. (0, 0, 0), the default, is off. Finite values
+ * only (EXG_ERR_INVALID otherwise). */
+int exg_world_set_uniform_field(exg_world* w, double ax, double ay, double az);
+
 /* ───────────── arenas: many independent worlds stepped together ─────────────
  * The batched / served form of a Driver epoch stepping several mechanisms
  * (crates/newton/src/driver.rs; accelerator/mod.rs:161-277, 613-630 flush their rows
