@@ -158,6 +158,7 @@ struct exg_world {
     double field[3] = {0.0, 0.0, 0.0}; // UniformField acceleration
     bool field_on = false;
     bool island_reduce_ok = false;    // every island on-chip and every tile within the islands' chunk cap
+    bool explicit_reduce_ok = false;  // every tile within the chunk cap (the explicit kernel)
     DevBuf<int> island_start, island_jstart, jkind, ja, jb, seeded, stats;
     DevBuf<int> fatal;              // island-steps that ended with a non-finite state
     DevBuf<int> requeue;            // single-body islands whose step subdivides (one per such island)
@@ -293,6 +294,7 @@ int exg_world_create(int n_bodies, const double* pose, const double* momentum, c
         bool chunks_ok = true;
         for (const auto& tl : tiles) chunks_ok &= tl.ne <= exg::GRAV_ISLAND_REDUCE_MAX_CHUNKS;
         w->island_reduce_ok = chunks_ok;   // and every island on-chip (checked with the classes below)
+        w->explicit_reduce_ok = chunks_ok;
         W_TRY(w->grav_stage.alloc(B * exg::GRAV_STAGE_BYTES / sizeof(double)));
     }
     W_TRY(w->island_start.alloc(I + 1));
@@ -474,10 +476,12 @@ static int enqueue_world_step(exg_world* w, double dt, double warp) {
     // timing events are recorded by the dispatches themselves (no marker packets)
     hipEvent_t* ev = w->step_ev;
     hipEvent_t g0 = ev ? ev[0] : nullptr, g1 = ev ? ev[1] : nullptr;
-    // the islands add the f64 partial sums themselves (the reduce kernel is skipped):
-    // every island in a 4- or 8-body class, the Newton integrator, f64 pairs
-    const bool island_reduce = w->gravity && w->grav_island_reduce && w->island_reduce_ok &&
-                               w->integrator == EXG_NEWTON && w->grav_prec == EXG_PREC_F64 && w->grav_entries.n > 0;
+    // the island kernels add the f64 partial sums themselves (the reduce kernel is
+    // skipped): under the Newton integrator every island in a 4- or 8-body class; the
+    // explicit kernel takes every island it takes
+    const bool island_reduce = w->gravity && w->grav_island_reduce && w->grav_prec == EXG_PREC_F64 &&
+                               w->grav_entries.n > 0 &&
+                               (w->integrator == EXG_NEWTON ? w->island_reduce_ok : w->explicit_reduce_ok);
     if (w->gravity)
         exg::launch_gravity(front, w->grav_entries.p, (int)w->grav_entries.n, w->grav_tiles.p, (int)w->grav_tiles.n,
                             w->grav_prec, w->grav_partial.p, w->ext.p, w->grav_stage.p, w->B, w->stream, g0, g1,

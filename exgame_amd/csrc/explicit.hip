@@ -77,9 +77,33 @@ __global__ __launch_bounds__(EXPL_THREADS) void explicit_step_kernel(IslandArgs 
         for (int k = 0; k < 8; ++k) pose[i][k] = a.pose[(size_t)(b0 + i) * 8 + k];
         for (int k = 0; k < 6; ++k) {
             mom[i][k] = a.mom[(size_t)(b0 + i) * 6 + k];
-            total[i][k] = field_add(a, a.mass[b0 + i], k, a.ext[(size_t)(b0 + i) * 6 + k]);
+            total[i][k] = a.grav_tiles ? 0.0 : field_add(a, a.mass[b0 + i], k, a.ext[(size_t)(b0 + i) * 6 + k]);
         }
         world_velocity(pose[i], a.mass[b0 + i], a.tens + (size_t)(b0 + i) * 9, mom[i], wv[i]);
+        if (a.grav_tiles) {
+            // the gravity pass's partial sums added here in place of its reduce kernel:
+            // the tile's chunks in ascending order and the same wrench arithmetic (the
+            // same bits); sixteen chunks' loads in flight at a time
+            const int gi = b0 + i;
+            const GravTile& tl = a.grav_tiles[gi / GRAV_TILE];
+            const double* p = a.grav_partial + (size_t)tl.e0 * GRAV_PARTIAL_PER_ENTRY + (gi - tl.tgt0);
+            double f[3] = {0.0, 0.0, 0.0};
+            for (int c0 = 0; c0 < tl.ne; c0 += 16) {
+                double v[16][3];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        v[u][k] = c0 + u < tl.ne ? p[(size_t)(c0 + u) * GRAV_PARTIAL_PER_ENTRY + k * GRAV_TILE] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (c0 + u < tl.ne)
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) f[k] += v[u][k];
+            }
+            gravity_wrench(a.grav_charges + (size_t)gi * 8, tl.g_inv, f, total[i]);
+            for (int k = 0; k < 6; ++k) total[i][k] = field_add(a, a.mass[gi], k, total[i][k]);
+        }
     }
     for (int q = 0; q < J; ++q) {
         int ia = a.ja[j0 + q] - b0, ib = a.jb[j0 + q] - b0;

@@ -287,9 +287,9 @@ int exg_world_set_precision(exg_world* w, int gravity, int jacobian);
 /* Who adds the gravity pass's partial sums (f64 pairs): with on = 1 (the default) the
  * island kernels add their own bodies' partial sums in the reduce kernel's order and
  * arithmetic (the same bits), and the reduce kernel is not launched; with on = 0 the
- * reduce kernel runs. Takes effect for worlds whose islands all have 2 to 8 bodies,
- * stepped by the Newton integrator with f64 gravity; the others run the reduce kernel
- * either way. No counterpart in the reference (a schedule of GravityPropagator's
+ * reduce kernel runs. Takes effect with f64 gravity for worlds stepped by the explicit
+ * integrators, and for worlds whose islands all have 2 to 8 bodies stepped by the
+ * Newton integrator; the others run the reduce kernel either way. No counterpart in the reference (a schedule of GravityPropagator's
  * accumulate, crates/newton/src/gravity.rs:530-601). */
 int exg_world_set_gravity_island_reduce(exg_world* w, int on);
 

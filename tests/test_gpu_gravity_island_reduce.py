@@ -1,5 +1,6 @@
 """The islands add the gravity pass's f64 partial sums themselves (the reduce kernel
-is not launched) for worlds whose islands all have 2 to 8 bodies. They add each body's
+is not launched): under Newton for worlds whose islands all have 2 to 8 bodies, under
+the explicit integrators for every world. They add each body's
 chunks in the reduce kernel's order with its wrench arithmetic, so a world steps to the
 same bits with the reduce kernel (exg_world_set_gravity_island_reduce(w, 0)) and
 without it, in every on-chip class, with FP32 Jacobian lanes and under graph replay.
@@ -13,13 +14,13 @@ from exgame_amd.world import DeviceWorld
 pytestmark = pytest.mark.gpu
 
 
-def same_bits(scene, steps, **kw):
+def same_bits(scene, steps, dt=1e-2, **kw):
     a = DeviceWorld(scene, tol2=1e-20, **kw)
     b = DeviceWorld(scene, tol2=1e-20, gravity_island_reduce=False,
                     **{k: v for k, v in kw.items() if k != "graph"})
     for n in steps:
-        a.step(n, 1e-2, 1.0)
-        b.step(n, 1e-2, 1.0)
+        a.step(n, dt, 1.0)
+        b.step(n, dt, 1.0)
         sa, sb = a.state(), b.state()
         for k in ("pose", "momentum", "origin"):
             assert np.array_equal(sa[k], sb[k]), (n, k)
@@ -46,3 +47,11 @@ def test_fp32_jacobian_lanes_same_bits():
 
 def test_graph_replay_same_bits_as_eager_with_the_reduce_kernel():
     same_bits(scenes.jointed_chains(n_chains=128, m=8, seed=1), (4, 2, 6), graph=True)
+
+
+@pytest.mark.parametrize("integrator", ["symplectic_euler", "explicit_euler", "lie_euler"])
+def test_explicit_integrators_same_bits(integrator):
+    # 1,024 point masses (the explicit configuration) and a mixed world with chains and
+    # rigs, stepped by the explicit kernel, which adds the partial sums itself
+    same_bits(scenes.plain_gravity(n=1024, seed=0), (5, 7), dt=1e-3, integrator=integrator)
+    same_bits(scenes.mixed_world(300, seed=4, rigs=True), (4,), dt=1e-3, integrator=integrator, graph=True)
