@@ -185,3 +185,21 @@ def test_arena_takes_kinematic_bodies_anywhere():
     a, b = ar.state(wid), orc.state()
     assert rel(a["position"], b["position"]) < 1e-12
     assert rel(a["momentum"], b["momentum"]) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_zero_size_anchors_in_both_gravity_precisions(precision):
+    # the reference's kinematic body has effective_size(0) = 0: as a pair's higher slot its
+    # core is 0 (1/s infinite; the law's min(1/d, 1/s) takes 1/d), and its zero charge
+    # pulls nothing; the mixed-precision records split 1/s = ∞ into hi + lo without harm
+    scene = scenes.anchored_chains(n_chains=16, m=8, seed=9)
+    size = scene.size.copy()
+    size[scene.mass == 0.0] = 0.0
+    object.__setattr__(scene, "size", size)
+    dev = DeviceWorld(scene, tol2=1e-20, gravity_precision=precision, jacobian_precision=precision)
+    orc = pyoracle.OracleWorld(scene, tol2=1e-20)
+    dev.step(20, 1e-2, 1.0)
+    orc.step(20, 1e-2, 1.0)
+    a, b = dev.state(), orc.state()
+    assert np.isfinite(a["pose"]).all() and dev.fatals() == 0
+    assert rel(a["position"], b["position"]) < (1e-12 if precision == "f64" else 1e-9)
